@@ -1,0 +1,203 @@
+"""Benchmark: all-pairs euclidean distances, n=65536 points x d=512 features.
+
+One step = one gt_dist call over the whole point set (inputs resident in HBM,
+packed FP32 lower-triangle output of n(n-1)/2 = 2,147,450,880 distances in
+HBM).  The headline precision (default: BF16 hi+lo split, three MFMA
+products, exact FP32 recompute of near-coincident pairs) and the plain FP32
+VALU kernel are both timed; value is the headline path's pair-distances/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3]
+
+Prints one JSON line.  The input is the 64-centre Gaussian mixture (spread
+1e-3) that stresses cancellation in the Gram form.  The cpu_baseline leg runs
+the FP64 CPU oracle on every pair among a seeded sample of rows, reports its
+throughput, and the same pairs give the max relative error of both GPU paths.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+N_POINTS = 65536
+N_FEATURES = 512
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32"])
+    ap.add_argument("--n", type=int, default=N_POINTS)
+    ap.add_argument("--k", type=int, default=N_FEATURES)
+    ap.add_argument("--fp32-steps", type=int, default=None,
+                    help="timed steps of the plain FP32 comparison (default: --steps)")
+    ap.add_argument("--sample-rows", type=int, default=4096,
+                    help="rows whose all pairs form the CPU-baseline / error sample")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def time_path(lib, x, out, ws, ws_bytes, precision, steps, warmup, torch):
+    """Median-free plain timing: W untimed steps, then K steps between two HIP
+    events recorded on the stream the kernels are launched on."""
+    from gputools_amd import _lib
+    n, k = x.shape
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream or None
+    prec = _lib.PRECISIONS[precision]
+
+    def step():
+        s = lib.gt_dist_ws(x.data_ptr(), n, k, x.stride(0), _lib.EUCLIDEAN, 2.0, prec,
+                           _lib.LAYOUT_PACKED, out.data_ptr(), n,
+                           ws.data_ptr() if ws is not None else None, ws_bytes, sp)
+        if s != 0:
+            raise RuntimeError(_lib.last_error())
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def sampled_pairs(n, rows):
+    """Packed indices of every pair among the (sorted) sample rows."""
+    r = np.sort(rows)
+    jj, ii = np.triu_indices(r.size, 1)      # local j < i
+    gi, gj = r[ii], r[jj]
+    idx = gj * n - gj * (gj + 1) // 2 + (gi - gj - 1)
+    return r, idx
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import gputools_amd as gt
+    from gputools_amd import workloads as wl
+
+    lib = gt.lib()
+    n, k = args.n, args.k
+    npairs = n * (n - 1) // 2
+    print("bench: generating %dx%d mixture" % (n, k), file=sys.stderr, flush=True)
+    xh = wl.mixture(n, k, seed=3)
+    x = torch.from_numpy(xh).cuda()
+    out = torch.empty(npairs, dtype=torch.float32, device="cuda")
+
+    results = {}
+    for precision in (args.precision, "fp32") if args.precision != "fp32" else ("fp32",):
+        ws_bytes = gt.workspace_size(n, k, "euclidean", precision)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda") if ws_bytes else None
+        steps = args.steps if precision == args.precision else (args.fp32_steps or args.steps)
+        print("bench: timing %s, %d warmup + %d steps" % (precision, args.warmup, steps),
+              file=sys.stderr, flush=True)
+        ms = time_path(lib, x, out, ws, ws_bytes, precision, steps, args.warmup, torch)
+        rec = {"ms_per_step": ms, "pairs_per_s": npairs / (ms * 1e-3)}
+        if not args.no_cpu_baseline:
+            rng = np.random.default_rng(12345)
+            rows = rng.choice(n, size=min(args.sample_rows, n), replace=False)
+            rec["_rows"], idx = sampled_pairs(n, rows)
+            rec["_sample"] = out[torch.from_numpy(idx).cuda()].cpu().numpy()
+        results[precision] = rec
+        print("bench: %s %.3f ms/step" % (precision, ms), file=sys.stderr, flush=True)
+
+    head = results[args.precision]
+    ms = head["ms_per_step"]
+    # Algorithmic work per launch: packed FP32 output (4 B/pair) plus one read
+    # of the n x k FP32 input; 2k flops/pair for the Gram form, 3k for the
+    # difference form.
+    bytes_per_launch = 4.0 * npairs + 4.0 * n * k
+    flops_per_launch = (2.0 if args.precision != "fp32" else 3.0) * k * npairs
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
+    tflops = flops_per_launch / (ms * 1e-3) / 1e12
+    t_hbm = bytes_per_launch / (HBM_PEAK_GBS * 1e9)
+    t_cmp = flops_per_launch / (PEAK_TFLOPS[args.precision] * 1e12)
+    if t_hbm >= t_cmp:
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    else:
+        roof = {"bound": "mfma" if args.precision != "fp32" else "valu",
+                "achieved": round(tflops, 2), "peak": PEAK_TFLOPS[args.precision],
+                "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS[args.precision], 4)}
+    roof["traffic"] = None
+    roof["kernel_ms"] = round(ms, 4)
+    roof["algorithmic_bytes"] = bytes_per_launch
+    roof["algorithmic_flops"] = flops_per_launch
+
+    line = {
+        "metric": "Euclidean pair-dists/s n=65536 d=512: BF16-split vs FP32, at <=1e-5 rel err",
+        "value": round(npairs / (ms * 1e-3), 1),
+        "unit": "pair-dists/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16", "fp16x3": "fp16", "fp32": "fp32"}[args.precision],
+        "data": "synthetic: 64-centre Gaussian mixture, spread 1e-3, numpy seed 3",
+        "config": {"workload": "gpuDist euclidean all pairs, n=%d x d=%d, packed dist output"
+                               % (n, k), "n": n, "d": k, "method": "euclidean",
+                   "precision": args.precision, "pairs": npairs},
+        "roofline": roof,
+    }
+    if args.precision != "fp32":
+        f = results["fp32"]
+        line["fp32_plain"] = {"ms_per_step": round(f["ms_per_step"], 4),
+                              "pairs_per_s": round(f["pairs_per_s"], 1)}
+        line["speedup_vs_fp32"] = round(f["ms_per_step"] / ms, 3)
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        rows = head["_rows"]
+        xs = np.ascontiguousarray(xh[rows])
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        print("bench: cpu baseline, oracle on %d rows, %d threads" % (rows.size, threads),
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ref = orc.dist(xs, "euclidean", nthreads=threads)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(ref.size / dt, 1), "unit": "pair-dists/s",
+                                "cores": threads, "kind": "port",
+                                "sample": "all %d pairs among %d seeded rows of the %dx%d "
+                                          "mixture, FP64 oracle, %.2f s"
+                                          % (ref.size, rows.size, n, k, dt)}
+        errs = {}
+        for precision, rec in results.items():
+            g = rec["_sample"].astype(np.float64)
+            errs[precision] = float(np.max(np.abs(g - ref) / ref))
+        line["max_rel_err"] = errs[args.precision]
+        line["max_rel_err_by_path"] = {p: float("%.3g" % e) for p, e in errs.items()}
+        line["error_sample_pairs"] = int(ref.size)
+
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

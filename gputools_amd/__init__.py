@@ -1,0 +1,13 @@
+"""gputools_amd -- MI355X-native pairwise distances (gputools' gpuDist path).
+
+The compute is in libgputools_amd.so (hand-written gfx950 HIP kernels behind a
+C-ABI, include/gputools_amd.h); this package is the host-side mirror of the
+reference's R interface plus torch-tensor device entry points.
+"""
+from ._lib import GputoolsError, LibraryNotBuilt, lib  # noqa: F401
+from .dist import (Dist, Rdistances, dist, dist_cross, dist_device_full,  # noqa: F401
+                   free_device, gpuDist, match_method, pmatch, workspace_size)
+
+__all__ = ["Dist", "GputoolsError", "LibraryNotBuilt", "Rdistances", "dist", "dist_cross",
+           "dist_device_full", "free_device", "gpuDist", "lib", "match_method", "pmatch",
+           "workspace_size"]

@@ -1,0 +1,511 @@
+// gt_capi.cpp -- the C-ABI of libgputools_amd.so (declared in
+// include/gputools_amd.h): argument checking, workspace handling and
+// dispatch to the gfx950 kernels.
+//
+// Replaces gputools' host driver (src/distance.cpp:209-344), its .C glue
+// (src/rinterface.cpp:79-85, 147-156, 172-214) and its NVRTC runtime
+// (src/cudaUtils.cpp:117-218): kernels are precompiled, nothing is JIT-compiled
+// or module-loaded per call, errors become status codes (or R errors when the
+// library runs inside R) instead of longjmps that leak device buffers.
+#include "../../include/gputools_amd.h"
+#include "gt_internal.h"
+
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <limits>
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(e == hipErrorOutOfMemory ? GT_ERR_ALLOC : GT_ERR_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define GT_HIP(call)                                                    \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);               \
+    } while (0)
+
+// When the library is loaded into an R process, a failed .C entry point
+// raises an R error (the convention of checkCudaError, src/cuseful.cpp:95-102);
+// otherwise the message stays available through gt_last_error().  Called only
+// after every resource of the entry point has been released, because Rf_error
+// does not return.
+void raise_in_r(int status)
+{
+    if (status == GT_OK) return;
+    typedef void (*rf_error_t)(const char *, ...);
+    static rf_error_t rf_error = (rf_error_t)dlsym(RTLD_DEFAULT, "Rf_error");
+    if (rf_error) rf_error("%s", g_err);
+}
+
+bool package_loaded = false;
+
+int check_method(int method, float p)
+{
+    if (method < GT_EUCLIDEAN || method > GT_MINKOWSKI)
+        return fail(GT_ERR_INVALID, "unknown distance method %d", method);
+    if (method == GT_MINKOWSKI && !(p > 0.0f && isfinite(p)))
+        return fail(GT_ERR_INVALID, "invalid p %g for minkowski (must be finite and > 0)",
+                    (double)p);
+    return GT_OK;
+}
+
+// Per-pair operation of the FP32 path for a method (minkowski p = 1, 2, 3 are
+// the manhattan, euclidean and cube specialisations).
+int plain_op(int method, float p)
+{
+    switch (method) {
+    case GT_EUCLIDEAN: return gt::OP_EUCLID;
+    case GT_MAXIMUM: return gt::OP_MAXIMUM;
+    case GT_MANHATTAN: return gt::OP_MANHATTAN;
+    case GT_CANBERRA: return gt::OP_CANBERRA;
+    case GT_MINKOWSKI:
+        if (p == 1.0f) return gt::OP_MANHATTAN;
+        if (p == 2.0f) return gt::OP_EUCLID;
+        if (p == 3.0f) return gt::OP_MINK3;
+        return gt::OP_MINKP;
+    default: return -1;
+    }
+}
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int workspace_bytes(int64_t n, int64_t k, int method, int precision, size_t *bytes)
+{
+    size_t b = 0;
+    if (k > 0 && n > 1) {
+        if (method == GT_BINARY)
+            b = round_up((size_t)n * (size_t)gt::binary_words(k) * sizeof(uint32_t), 256);
+        else if (precision != GT_PREC_FP32)
+            b = gt::gram_workspace_bytes(n, k);
+    }
+    *bytes = b;
+    return GT_OK;
+}
+
+int check_dist_args(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+                    int precision, int layout, const float *out, int64_t ldd)
+{
+    int s = check_method(method, p);
+    if (s != GT_OK) return s;
+    if (n < 0 || k < 0) return fail(GT_ERR_INVALID, "negative size (n=%lld, k=%lld)",
+                                    (long long)n, (long long)k);
+    if (n > (int64_t)std::numeric_limits<int>::max() / 2)
+        return fail(GT_ERR_INVALID, "n=%lld exceeds the supported maximum", (long long)n);
+    if (ldx < k) return fail(GT_ERR_INVALID, "ldx=%lld < k=%lld", (long long)ldx, (long long)k);
+    if (layout != GT_LAYOUT_PACKED && layout != GT_LAYOUT_FULL)
+        return fail(GT_ERR_INVALID, "unknown layout %d", layout);
+    if (layout == GT_LAYOUT_FULL && ldd < n)
+        return fail(GT_ERR_INVALID, "ldd=%lld < n=%lld", (long long)ldd, (long long)n);
+    if (precision < GT_PREC_FP32 || precision > GT_PREC_FP16X3)
+        return fail(GT_ERR_INVALID, "unknown precision %d", precision);
+    if (precision != GT_PREC_FP32 && method != GT_EUCLIDEAN)
+        return fail(GT_ERR_UNSUPPORTED,
+                    "split-precision Gram form exists for euclidean only (method %d)", method);
+    const bool has_out = (layout == GT_LAYOUT_PACKED) ? n > 1 : n > 0;
+    if (has_out && out == nullptr) return fail(GT_ERR_INVALID, "null output pointer");
+    if (has_out && k > 0 && x == nullptr) return fail(GT_ERR_INVALID, "null input pointer");
+    return GT_OK;
+}
+
+int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+             int precision, int layout, float *out, int64_t ldd, void *ws, hipStream_t s)
+{
+    const int glayout = (layout == GT_LAYOUT_PACKED) ? gt::OUT_PACKED : gt::OUT_FULL;
+    if (layout == GT_LAYOUT_PACKED && n < 2) return GT_OK;
+    if (n < 1) return GT_OK;
+    if (k == 0) {
+        // No usable feature: R returns NA for every pair.
+        const float na = __builtin_nanf("");
+        if (layout == GT_LAYOUT_PACKED)
+            GT_HIP(gt::launch_fill(out, n * (n - 1) / 2, na, s));
+        else
+            GT_HIP(gt::launch_fill_full(out, n, ldd, na, s));
+        return GT_OK;
+    }
+    if (method == GT_BINARY) {
+        if (n == 1) {
+            GT_HIP(gt::launch_fill_full(out, 1, ldd, 0.0f, s));
+            return GT_OK;
+        }
+        uint32_t *bits = static_cast<uint32_t *>(ws);
+        const int64_t words = gt::binary_words(k);
+        GT_HIP(gt::launch_binary_pack(x, n, k, ldx, bits, words, s));
+        GT_HIP(gt::launch_binary(glayout, bits, n, bits, n, words, words, out, ldd, s));
+        return GT_OK;
+    }
+    if (precision != GT_PREC_FP32) {
+        GT_HIP(gt::launch_gram(precision, glayout, x, n, k, ldx, out, ldd, ws, s));
+        return GT_OK;
+    }
+    GT_HIP(gt::launch_plain(plain_op(method, p), glayout, x, n, ldx, x, n, ldx, k, p, out, ldd, s));
+    return GT_OK;
+}
+
+// Stream-ordered scratch: hipMallocAsync / hipFreeAsync on the caller's stream
+// (safe against concurrent calls on other streams).  Falls back to a
+// synchronous hipMalloc/hipFree pair where the pool is unavailable.
+struct Scratch {
+    void *ptr = nullptr;
+    bool async = false;
+    hipStream_t stream = nullptr;
+
+    int acquire(size_t bytes, hipStream_t s)
+    {
+        stream = s;
+        if (bytes == 0) return GT_OK;
+        hipError_t e = hipMallocAsync(&ptr, bytes, s);
+        if (e == hipSuccess) {
+            async = true;
+            return GT_OK;
+        }
+        (void)hipGetLastError();
+        ptr = nullptr;
+        e = hipMalloc(&ptr, bytes);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return hip_fail(e, "hipMalloc(workspace)");
+        }
+        return GT_OK;
+    }
+
+    int release()
+    {
+        if (!ptr) return GT_OK;
+        hipError_t e;
+        if (async) {
+            e = hipFreeAsync(ptr, stream);
+        } else {
+            e = hipStreamSynchronize(stream);
+            hipError_t e2 = hipFree(ptr);
+            if (e == hipSuccess) e = e2;
+        }
+        ptr = nullptr;
+        return e == hipSuccess ? GT_OK : hip_fail(e, "releasing workspace");
+    }
+};
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    ~DeviceBuffer()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+    int alloc(size_t bytes, const char *what)
+    {
+        if (bytes == 0) return GT_OK;
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return hip_fail(e, what);
+        }
+        return GT_OK;
+    }
+};
+
+int dist_method_from_name(const char *name)
+{
+    // getDistEnum (src/rinterface.cpp:147-156): unknown names -> euclidean.
+    if (name == nullptr) return GT_EUCLIDEAN;
+    if (strcmp(name, "maximum") == 0) return GT_MAXIMUM;
+    if (strcmp(name, "manhattan") == 0) return GT_MANHATTAN;
+    if (strcmp(name, "canberra") == 0) return GT_CANBERRA;
+    if (strcmp(name, "binary") == 0) return GT_BINARY;
+    if (strcmp(name, "minkowski") == 0) return GT_MINKOWSKI;
+    return GT_EUCLIDEAN;
+}
+
+int dist_host(const float *x_host, int64_t n, int64_t k, int method, float p, int precision,
+              int layout, float *out_host)
+{
+    const int64_t ldd = n;
+    int s = check_dist_args(x_host, n, k, k, method, p, precision, layout, out_host, ldd);
+    if (s != GT_OK) return s;
+    const size_t out_count =
+        (layout == GT_LAYOUT_PACKED) ? (size_t)(n > 1 ? n * (n - 1) / 2 : 0) : (size_t)(n * n);
+    if (out_count == 0) return GT_OK;
+    size_t ws_bytes = 0;
+    workspace_bytes(n, k, method, precision, &ws_bytes);
+    DeviceBuffer dx, dout, dws;
+    if ((s = dx.alloc((size_t)n * (size_t)k * sizeof(float), "hipMalloc(points)")) != GT_OK) return s;
+    if ((s = dout.alloc(out_count * sizeof(float), "hipMalloc(distances)")) != GT_OK) return s;
+    if ((s = dws.alloc(ws_bytes, "hipMalloc(workspace)")) != GT_OK) return s;
+    if (k > 0)
+        GT_HIP(hipMemcpy(dx.ptr, x_host, (size_t)n * (size_t)k * sizeof(float),
+                         hipMemcpyHostToDevice));
+    s = run_dist(static_cast<const float *>(dx.ptr), n, k, k, method, p, precision, layout,
+                 static_cast<float *>(dout.ptr), ldd, dws.ptr, nullptr);
+    if (s != GT_OK) return s;
+    GT_HIP(hipMemcpy(out_host, dout.ptr, out_count * sizeof(float), hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gt_last_error(void) { return g_err; }
+
+int gt_version(void) { return 10000; }
+
+int gt_method_from_name(const char *name) { return dist_method_from_name(name); }
+
+int gt_dist_workspace_size(int64_t n, int64_t k, int method, int precision, size_t *bytes)
+{
+    if (bytes == nullptr) return fail(GT_ERR_INVALID, "null bytes pointer");
+    if (n < 0 || k < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (method < GT_EUCLIDEAN || method > GT_MINKOWSKI)
+        return fail(GT_ERR_INVALID, "unknown distance method %d", method);
+    return workspace_bytes(n, k, method, precision, bytes);
+}
+
+int gt_dist_ws(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+               int precision, int layout, float *out, int64_t ldd, void *workspace,
+               size_t workspace_bytes_given, void *stream)
+{
+    int s = check_dist_args(x, n, k, ldx, method, p, precision, layout, out, ldd);
+    if (s != GT_OK) return s;
+    size_t need = 0;
+    workspace_bytes(n, k, method, precision, &need);
+    if (need > 0 && (workspace == nullptr || workspace_bytes_given < need))
+        return fail(GT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes_given);
+    return run_dist(x, n, k, ldx, method, p, precision, layout, out, ldd, workspace,
+                    static_cast<hipStream_t>(stream));
+}
+
+int gt_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+            int precision, int layout, float *out, int64_t ldd, void *stream)
+{
+    int s = check_dist_args(x, n, k, ldx, method, p, precision, layout, out, ldd);
+    if (s != GT_OK) return s;
+    size_t need = 0;
+    workspace_bytes(n, k, method, precision, &need);
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    Scratch ws;
+    if ((s = ws.acquire(need, hs)) != GT_OK) return s;
+    s = run_dist(x, n, k, ldx, method, p, precision, layout, out, ldd, ws.ptr, hs);
+    const int r = ws.release();
+    return s != GT_OK ? s : r;
+}
+
+int gt_dist_cross(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
+                  int64_t ldb, int64_t k, int method, float p, float *out, int64_t ldd,
+                  void *stream)
+{
+    int s = check_method(method, p);
+    if (s != GT_OK) return s;
+    if (na < 0 || nb < 0 || k < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (lda < k || ldb < k) return fail(GT_ERR_INVALID, "leading dimension < k");
+    if (ldd < na) return fail(GT_ERR_INVALID, "ldd < na");
+    if (na == 0 || nb == 0) return GT_OK;
+    if (out == nullptr || (k > 0 && (a == nullptr || b == nullptr)))
+        return fail(GT_ERR_INVALID, "null pointer");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    if (k == 0) {
+        for (int64_t y = 0; y < nb; ++y) GT_HIP(gt::launch_fill(out + y * ldd, na, __builtin_nanf(""), hs));
+        return GT_OK;
+    }
+    if (method == GT_BINARY) {
+        const int64_t words = gt::binary_words(k);
+        const size_t abytes = round_up((size_t)na * words * sizeof(uint32_t), 256);
+        const size_t bbytes = round_up((size_t)nb * words * sizeof(uint32_t), 256);
+        Scratch ws;
+        if ((s = ws.acquire(abytes + bbytes, hs)) != GT_OK) return s;
+        uint32_t *abits = static_cast<uint32_t *>(ws.ptr);
+        uint32_t *bbits = reinterpret_cast<uint32_t *>(static_cast<char *>(ws.ptr) + abytes);
+        hipError_t e = gt::launch_binary_pack(a, na, k, lda, abits, words, hs);
+        if (e == hipSuccess) e = gt::launch_binary_pack(b, nb, k, ldb, bbits, words, hs);
+        if (e == hipSuccess)
+            e = gt::launch_binary(gt::OUT_CROSS, abits, na, bbits, nb, words, words, out, ldd, hs);
+        const int r = ws.release();
+        if (e != hipSuccess) return hip_fail(e, "binary cross launch");
+        return r;
+    }
+    GT_HIP(gt::launch_plain(plain_op(method, p), gt::OUT_CROSS, a, na, lda, b, nb, ldb, k, p, out,
+                            ldd, hs));
+    return GT_OK;
+}
+
+int gt_dist_host(const float *x_host, int64_t n, int64_t k, int method, float p, int precision,
+                 int layout, float *out_host)
+{
+    return dist_host(x_host, n, k, method, p, precision, layout, out_host);
+}
+
+int gt_dist_device_full(const float *x_host, int64_t n, int64_t k, int method, float p,
+                        float **d_out, size_t *pitch_bytes)
+{
+    if (d_out == nullptr || pitch_bytes == nullptr) return fail(GT_ERR_INVALID, "null output pointer");
+    *d_out = nullptr;
+    *pitch_bytes = 0;
+    int s = check_dist_args(x_host, n, k, k, method, p, GT_PREC_FP32, GT_LAYOUT_FULL,
+                            reinterpret_cast<const float *>(d_out), n);
+    if (s != GT_OK) return s;
+    if (n == 0) return GT_OK;
+    size_t ws_bytes = 0;
+    workspace_bytes(n, k, method, GT_PREC_FP32, &ws_bytes);
+    DeviceBuffer dx, dws;
+    void *d = nullptr;
+    size_t pitch = 0;
+    if ((s = dx.alloc((size_t)n * (size_t)k * sizeof(float), "hipMalloc(points)")) != GT_OK) return s;
+    if ((s = dws.alloc(ws_bytes, "hipMalloc(workspace)")) != GT_OK) return s;
+    // Pitched n x n result, as distanceLeaveOnGpu allocates it (cudaMallocPitch).
+    hipError_t e = hipMallocPitch(&d, &pitch, (size_t)n * sizeof(float), (size_t)n);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocPitch(distances)");
+    if (k > 0) {
+        e = hipMemcpy(dx.ptr, x_host, (size_t)n * (size_t)k * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return hip_fail(e, "hipMemcpy(points)");
+        }
+    }
+    s = run_dist(static_cast<const float *>(dx.ptr), n, k, k, method, p, GT_PREC_FP32,
+                 GT_LAYOUT_FULL, static_cast<float *>(d), (int64_t)(pitch / sizeof(float)),
+                 dws.ptr, nullptr);
+    if (s == GT_OK) {
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) s = hip_fail(e, "distance kernel");
+    }
+    if (s != GT_OK) {
+        (void)hipFree(d);
+        return s;
+    }
+    *d_out = static_cast<float *>(d);
+    *pitch_bytes = pitch;
+    return GT_OK;
+}
+
+int gt_free(void *device_ptr)
+{
+    if (device_ptr == nullptr) return GT_OK;
+    GT_HIP(hipFree(device_ptr));
+    return GT_OK;
+}
+
+// --------------------------- .C drop-in exports ---------------------------
+
+void Rdistances(const float *points, const int *numPoints, const int *dim, float *distances,
+                const char **method, const float *p)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!numPoints || !dim || !method || !p) {
+        s = fail(GT_ERR_INVALID, "Rdistances: null argument");
+    } else {
+        const int m = dist_method_from_name(*method);
+        s = dist_host(points, *numPoints, *dim, m, *p, GT_PREC_FP32, GT_LAYOUT_FULL, distances);
+    }
+    raise_in_r(s);
+}
+
+void Rdistclust(const char **distmethod, const char **clustmethod, const float *points,
+                const int *numPoints, const int *dim, int *merge, int *order, float *val)
+{
+    g_err[0] = '\0';
+    (void)clustmethod;
+    (void)merge;
+    (void)order;
+    (void)val;
+    int s;
+    if (!distmethod || !numPoints || !dim) {
+        s = fail(GT_ERR_INVALID, "Rdistclust: null argument");
+    } else {
+        // Distance stage, as Rdistclust runs it (src/rinterface.cpp:182-183):
+        // device-resident full matrix, p fixed at 2.
+        float *d = nullptr;
+        size_t pitch = 0;
+        s = gt_dist_device_full(points, *numPoints, *dim, dist_method_from_name(*distmethod), 2.0f,
+                                &d, &pitch);
+        if (d) (void)hipFree(d);
+        if (s == GT_OK)
+            s = fail(GT_ERR_UNSUPPORTED,
+                     "Rdistclust: the hierarchical-clustering stage is not part of this library");
+    }
+    raise_in_r(s);
+}
+
+void cuCompile(const int *numFiles, const char **cuFilenames, const char **cuSrc)
+{
+    (void)numFiles;
+    (void)cuFilenames;
+    (void)cuSrc;
+    package_loaded = true;
+}
+
+void unloadPackage(void) { package_loaded = false; }
+
+void rsetDevice(const int *device)
+{
+    g_err[0] = '\0';
+    int s = GT_OK;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) {
+        s = hip_fail(e, "hipGetDeviceCount");
+    } else if (device == nullptr || *device < 0 || *device >= count) {
+        s = fail(GT_ERR_INVALID, "The gpu id number is not valid.");
+    } else {
+        e = hipSetDevice(*device);
+        if (e != hipSuccess) s = hip_fail(e, "setDevice function : choosing gpu");
+    }
+    raise_in_r(s);
+}
+
+void rgetDevice(int *device)
+{
+    g_err[0] = '\0';
+    int s = GT_OK;
+    if (device == nullptr) {
+        s = fail(GT_ERR_INVALID, "rgetDevice: null argument");
+    } else {
+        hipError_t e = hipGetDevice(device);
+        if (e != hipSuccess) s = hip_fail(e, "hipGetDevice");
+    }
+    raise_in_r(s);
+}
+
+}  // extern "C"
+
+// ------------------------- not yet implemented ----------------------------
+extern "C" {
+int gt_dist_backward(const float *, int64_t, int64_t, int64_t, int, float, const float *, float *,
+                     int64_t, void *)
+{
+    return fail(GT_ERR_UNSUPPORTED, "gt_dist_backward: not built yet");
+}
+int gt_mds_stress(const float *, int64_t, int64_t, int64_t, const float *, float *, int64_t,
+                  double *, void *)
+{
+    return fail(GT_ERR_UNSUPPORTED, "gt_mds_stress: not built yet");
+}
+int gt_dist_batched(const float *, const int64_t *, const int64_t *, const int64_t *, int64_t,
+                    float *, const int64_t *, int, int, void *, size_t, void *)
+{
+    return fail(GT_ERR_UNSUPPORTED, "gt_dist_batched: not built yet");
+}
+int gt_dist_batched_workspace_size(const int64_t *, const int64_t *, int64_t, int, size_t *)
+{
+    return fail(GT_ERR_UNSUPPORTED, "gt_dist_batched: not built yet");
+}
+int gt_batched_tile_count(void) { return 0; }
+}

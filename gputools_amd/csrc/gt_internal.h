@@ -1,0 +1,60 @@
+// gt_internal.h -- host-side declarations shared by the library's translation
+// units (kernel launchers and the C-ABI layer).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gt {
+
+// Per-pair accumulation of the FP32 VALU path.
+enum PlainOp {
+    OP_EUCLID = 0,     // sum (a-b)^2, then sqrt            (distance.cu:40-78)
+    OP_MAXIMUM = 1,    // max |a-b|                          (distance.cu:116-154)
+    OP_MANHATTAN = 2,  // sum |a-b|                          (distance.cu:192-232)
+    OP_CANBERRA = 3,   // sum |a-b|/|a+b|, R's 0/0 rule       (distance.cu:273-316)
+    OP_MINK3 = 4,      // sum |a-b|^3, then cube root        (distance.cu:453-493, p = 3)
+    OP_MINKP = 5       // sum |a-b|^p, then ^(1/p)           (distance.cu:453-493)
+};
+
+enum OutLayout {
+    OUT_PACKED = 0,  // R 'dist' order
+    OUT_FULL = 1,    // symmetric n x n, zero diagonal
+    OUT_CROSS = 2    // nb x na, pair (x in A, y in B) at y*ldd + x
+};
+
+// Plain FP32 tile kernels.  Self mode: a == b, na == nb == n, tiles on and
+// below the diagonal.  Cross mode: every (A tile, B tile).
+hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t lda,
+                        const float *b, int64_t nb, int64_t ldb, int64_t k, float p,
+                        float *out, int64_t ldd, hipStream_t stream);
+
+// Binary method: rows -> bit rows (x != 0), then popcount tiles.
+int64_t binary_words(int64_t k);
+hipError_t launch_binary_pack(const float *x, int64_t n, int64_t k, int64_t ldx,
+                              uint32_t *bits, int64_t ldw, hipStream_t stream);
+hipError_t launch_binary(int layout, const uint32_t *a, int64_t na, const uint32_t *b,
+                         int64_t nb, int64_t words, int64_t ldw, float *out, int64_t ldd,
+                         hipStream_t stream);
+
+// Fill n floats with a value (used for the k == 0 case: every distance NA).
+hipError_t launch_fill(float *out, int64_t count, float value, hipStream_t stream);
+hipError_t launch_fill_full(float *out, int64_t n, int64_t ldd, float offdiag,
+                            hipStream_t stream);
+
+// Split-precision MFMA Gram path (euclidean).  Workspace: row norms (n floats).
+size_t gram_workspace_bytes(int64_t n, int64_t k);
+hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int64_t k,
+                       int64_t ldx, float *out, int64_t ldd, void *workspace,
+                       hipStream_t stream);
+
+// Backward (dL/dX) and fused MDS stress.
+hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t ldx,
+                           float p, const float *g_packed, float *dx, int64_t lddx,
+                           hipStream_t stream);
+hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy,
+                      const float *delta, float *dy, int64_t lddy, double *stress,
+                      hipStream_t stream);
+
+}  // namespace gt

@@ -1,0 +1,297 @@
+// gt_plain.hip -- FP32 all-pairs distance tiles on the gfx950 VALU.
+//
+// Replaces the six *_kernel_same kernels of inst/cuda/distance.cu (and the
+// unreachable cross kernels).  gputools runs one 32-thread block per pair,
+// re-reads both rows from global memory for every pair and reduces through
+// shared memory (distance.cu:56-69).  Here a 256-thread workgroup owns a
+// 128 x 128 tile of pairs: both 128-point row blocks are staged through LDS in
+// 16-feature chunks (k-major, so each lane reads float4 runs), and every lane
+// keeps an 8 x 8 register micro-tile of pair accumulators.  Each staged
+// feature value is reused by 128 pairs.
+//
+// Summation is two-level: each 16-feature chunk is summed into a partial that
+// is then added into the running total, which bounds the rounding error of a
+// sum of k positive terms by about (16 + k/16) ulp instead of k ulp.
+#include "gt_common.h"
+#include "gt_internal.h"
+
+#include <math.h>
+
+namespace gt {
+namespace {
+
+constexpr int PBM = 128;       // points per tile side
+constexpr int PBK = 16;        // features per LDS chunk
+constexpr int PTHREADS = 256;  // 4 waves
+constexpr int PLD = PBM + 4;   // LDS row length (padding against bank conflicts)
+constexpr int PLOADS = PBM * PBK / PTHREADS;  // elements each lane stages per side
+
+struct PlainArgs {
+    const float *a;  // column side (index i): self mode = x
+    int64_t lda, na;
+    const float *b;  // row side (index j): self mode = x
+    int64_t ldb, nb;
+    int64_t k;
+    float *out;
+    int64_t ldd;
+    int64_t tiles_a;      // cross mode: tiles along A
+    int64_t tile_offset;  // first logical tile of this launch
+    int64_t tiles_this;   // tiles in this launch
+    float p, inv_p, fk;
+};
+
+template <int OP>
+struct Term {
+    // Accumulate one feature of one pair into (part, cnt).
+    static __device__ __forceinline__ void add(float &part, int &cnt, float a, float b, float p)
+    {
+        if (OP == OP_EUCLID) {
+            const float t = a - b;
+            part = fmaf(t, t, part);
+        } else if (OP == OP_MAXIMUM) {
+            part = fmaxf(part, fabsf(a - b));
+        } else if (OP == OP_MANHATTAN) {
+            part += fabsf(a - b);
+        } else if (OP == OP_CANBERRA) {
+            // R's rule: a term with 0 numerator and 0 denominator (a == b == 0,
+            // the only case where the quotient is NaN for finite input) is
+            // omitted and counted as missing; a == -b != 0 gives x/0 = Inf.
+            const float q = fabsf(a - b) / fabsf(a + b);
+            const bool ok = (q == q);
+            part += ok ? q : 0.0f;
+            cnt += ok ? 1 : 0;
+        } else if (OP == OP_MINK3) {
+            const float t = fabsf(a - b);
+            part = fmaf(t * t, t, part);
+        } else {  // OP_MINKP: |t|^p = 2^(p log2|t|); log2(0) = -inf gives 0
+            const float t = fabsf(a - b);
+            part += exp2f(p * log2f(t));
+        }
+    }
+
+    static __device__ __forceinline__ float finish(float acc, int cnt, float inv_p, float fk)
+    {
+        if (OP == OP_EUCLID) return sqrtf(acc);
+        if (OP == OP_MAXIMUM || OP == OP_MANHATTAN) return acc;
+        if (OP == OP_CANBERRA) {
+            if (cnt == 0) return __builtin_nanf("");
+            return ((float)cnt == fk) ? acc : acc / ((float)cnt / fk);
+        }
+        if (OP == OP_MINK3) return cbrtf(acc);
+        return powf(acc, inv_p);
+    }
+};
+
+template <int OP, int LAYOUT>
+__global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
+{
+    __shared__ float lds[2][PBK][PLD];  // [0] column side (i), [1] row side (j)
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int64_t t = args.tile_offset + xcd_remap(blockIdx.x, args.tiles_this);
+
+    int I, J;
+    if (LAYOUT == OUT_CROSS) {
+        I = (int)(t % args.tiles_a);
+        J = (int)(t / args.tiles_a);
+    } else {
+        tri_tile(t, I, J);
+    }
+    const int64_t i0 = (int64_t)I * PBM, j0 = (int64_t)J * PBM;
+    const int64_t k = args.k;
+
+    // Global -> register staging (prefetched one chunk ahead).
+    float sc[PLOADS], sr[PLOADS];
+    auto gload = [&](int64_t kb) {
+#pragma unroll
+        for (int q = 0; q < PLOADS; ++q) {
+            const int e = tid + PTHREADS * q;
+            const int row = e >> 4, kc = e & 15;
+            const int64_t kk = kb + kc;
+            const int64_t gi = i0 + row, gj = j0 + row;
+            sc[q] = (gi < args.na && kk < k) ? args.a[gi * args.lda + kk] : 0.0f;
+            sr[q] = (gj < args.nb && kk < k) ? args.b[gj * args.ldb + kk] : 0.0f;
+        }
+    };
+
+    float acc[8][8];
+    float part[8][8];
+    int cnt[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            acc[r][c] = 0.0f;
+            part[r][c] = 0.0f;
+            cnt[r][c] = 0;
+        }
+
+    gload(0);
+    for (int64_t kb = 0; kb < k; kb += PBK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PLOADS; ++q) {
+            const int e = tid + PTHREADS * q;
+            const int row = e >> 4, kc = e & 15;
+            lds[0][kc][row] = sc[q];
+            lds[1][kc][row] = sr[q];
+        }
+        __syncthreads();
+        if (kb + PBK < k) gload(kb + PBK);
+
+#pragma unroll 4
+        for (int kc = 0; kc < PBK; ++kc) {
+            const float4 c0 = *reinterpret_cast<const float4 *>(&lds[0][kc][tx * 4]);
+            const float4 c1 = *reinterpret_cast<const float4 *>(&lds[0][kc][64 + tx * 4]);
+            const float4 r0 = *reinterpret_cast<const float4 *>(&lds[1][kc][ty * 4]);
+            const float4 r1 = *reinterpret_cast<const float4 *>(&lds[1][kc][64 + ty * 4]);
+            const float cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            const float rv[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) Term<OP>::add(part[r][c], cnt[r][c], cv[c], rv[r], args.p);
+        }
+        if (OP != OP_MAXIMUM) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    acc[r][c] += part[r][c];
+                    part[r][c] = 0.0f;
+                }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
+        if (j >= args.nb) continue;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+            if (i >= args.na) continue;
+            const float total = (OP == OP_MAXIMUM) ? part[r][c] : acc[r][c];
+            if (LAYOUT == OUT_PACKED) {
+                if (i > j)
+                    args.out[packed_base(args.na, j) + i] =
+                        Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+            } else if (LAYOUT == OUT_FULL) {
+                if (i > j) {
+                    const float v = Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+                    args.out[j * args.ldd + i] = v;
+                    args.out[i * args.ldd + j] = v;
+                } else if (i == j) {
+                    args.out[i * args.ldd + i] = 0.0f;
+                }
+            } else {
+                args.out[j * args.ldd + i] = Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+            }
+        }
+    }
+}
+
+template <int OP>
+hipError_t launch_op(int layout, PlainArgs &args, int64_t total, hipStream_t stream)
+{
+    // Keep each launch's thread count below 2^32.
+    const int64_t max_tiles = int64_t(1) << 23;
+    for (int64_t off = 0; off < total; off += max_tiles) {
+        args.tile_offset = off;
+        args.tiles_this = (total - off) < max_tiles ? (total - off) : max_tiles;
+        dim3 grid((unsigned)args.tiles_this), block(PTHREADS);
+        if (layout == OUT_PACKED)
+            hipLaunchKernelGGL((plain_tile_kernel<OP, OUT_PACKED>), grid, block, 0, stream, args);
+        else if (layout == OUT_FULL)
+            hipLaunchKernelGGL((plain_tile_kernel<OP, OUT_FULL>), grid, block, 0, stream, args);
+        else
+            hipLaunchKernelGGL((plain_tile_kernel<OP, OUT_CROSS>), grid, block, 0, stream, args);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t lda,
+                        const float *b, int64_t nb, int64_t ldb, int64_t k, float p,
+                        float *out, int64_t ldd, hipStream_t stream)
+{
+    PlainArgs args;
+    args.a = a;
+    args.lda = lda;
+    args.na = na;
+    args.b = b;
+    args.ldb = ldb;
+    args.nb = nb;
+    args.k = k;
+    args.out = out;
+    args.ldd = ldd;
+    args.p = p;
+    args.inv_p = 1.0f / p;
+    args.fk = (float)k;
+    const int64_t ta = (na + PBM - 1) / PBM, tb = (nb + PBM - 1) / PBM;
+    int64_t total;
+    if (layout == OUT_CROSS) {
+        args.tiles_a = ta;
+        total = ta * tb;
+    } else {
+        args.tiles_a = ta;
+        total = ta * (ta + 1) / 2;
+    }
+    if (total == 0) return hipSuccess;
+    switch (op) {
+    case OP_EUCLID: return launch_op<OP_EUCLID>(layout, args, total, stream);
+    case OP_MAXIMUM: return launch_op<OP_MAXIMUM>(layout, args, total, stream);
+    case OP_MANHATTAN: return launch_op<OP_MANHATTAN>(layout, args, total, stream);
+    case OP_CANBERRA: return launch_op<OP_CANBERRA>(layout, args, total, stream);
+    case OP_MINK3: return launch_op<OP_MINK3>(layout, args, total, stream);
+    case OP_MINKP: return launch_op<OP_MINKP>(layout, args, total, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fills: k == 0 gives NA for every pair (R: no usable column -> NA_REAL).
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void fill_kernel(float *out, int64_t count, float value)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x)
+        out[e] = value;
+}
+
+__global__ void fill_full_kernel(float *out, int64_t n, int64_t ldd, float offdiag)
+{
+    const int64_t total = n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / n, c = e % n;
+        out[r * ldd + c] = (r == c) ? 0.0f : offdiag;
+    }
+}
+}  // namespace
+
+hipError_t launch_fill(float *out, int64_t count, float value, hipStream_t stream)
+{
+    if (count <= 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, out, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_full(float *out, int64_t n, int64_t ldd, float offdiag, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n * n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_full_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, out, n, ldd,
+                       offdiag);
+    return hipGetLastError();
+}
+
+}  // namespace gt

@@ -1,0 +1,196 @@
+/*
+ * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in .C exports with the exact names and signatures that gputools'
+ *     R code binds through useDynLib(gputools) (NAMESPACE:1) and .C(...).
+ *     Every argument is a pointer to R-owned host memory; nothing is kept
+ *     after return.  Errors are raised as R errors (Rf_error) when the library
+ *     is loaded into an R process, and otherwise recorded for gt_last_error().
+ *
+ *  2. The native gt_* API: integer status return (GT_OK == 0), 64-bit sizes,
+ *     explicit leading dimensions (in elements), device pointers and an
+ *     optional HIP stream (NULL = the legacy default stream).  It never throws
+ *     or longjmps across the boundary.  Calls on distinct streams are
+ *     thread-safe.
+ *
+ * Layouts
+ *   X       row-major n x k FP32, row stride ldx >= k elements (the pitched
+ *           layout of src/distance.h:10-19 with the pitch in elements).
+ *   PACKED  n(n-1)/2 FP32, R 'dist' order: pair (i, j), i > j, 0-based, at
+ *           j*n - j*(j+1)/2 + (i - j - 1)  (what as.dist() keeps at
+ *           R/gpuHclust.R:29).
+ *   FULL    symmetric n x n FP32, row stride ldd >= n, zero diagonal
+ *           (what distance_device fills, src/distance.cpp:209-257).
+ *   CROSS   nb x na FP32, row stride ldd >= na: pair (x in A, y in B) at
+ *           y*ldd + x (the layout of the *_kernel cross kernels,
+ *           inst/cuda/distance.cu:3-38).
+ */
+#ifndef GPUTOOLS_AMD_H
+#define GPUTOOLS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Distance methods: same values as dist_method (src/distance.h:5-8). */
+enum gt_method {
+    GT_EUCLIDEAN = 0,
+    GT_MAXIMUM = 1,
+    GT_MANHATTAN = 2,
+    GT_CANBERRA = 3,
+    GT_BINARY = 4,
+    GT_MINKOWSKI = 5
+};
+
+/* Arithmetic of the euclidean path.  FP32 = exact differences on the VALU
+ * (every method).  BF16X3 / FP16X3 = Gram form ||x||^2 + ||y||^2 - 2 x.y on
+ * MFMA with a hi+lo split (three products, FP32 accumulation) and exact FP32
+ * recomputation of near-coincident pairs; euclidean only. */
+enum gt_precision {
+    GT_PREC_FP32 = 0,
+    GT_PREC_BF16X3 = 1,
+    GT_PREC_FP16X3 = 2
+};
+
+enum gt_layout {
+    GT_LAYOUT_PACKED = 0,
+    GT_LAYOUT_FULL = 1
+};
+
+enum gt_status {
+    GT_OK = 0,
+    GT_ERR_INVALID = 1,     /* bad argument: method, size, pointer, p */
+    GT_ERR_HIP = 2,         /* a HIP runtime call failed */
+    GT_ERR_ALLOC = 3,       /* device or host allocation failed */
+    GT_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+    GT_ERR_WORKSPACE = 5    /* caller workspace too small */
+};
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in .C exports                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces Rdistances (src/rinterface.cpp:206-214, decl src/rinterface.h:53-55).
+ * Fills the full symmetric numPoints x numPoints matrix 'distances' (host),
+ * as distance() does (src/distance.cpp:259-316).  Unknown method names select
+ * euclidean, as getDistEnum does (src/rinterface.cpp:147-156). */
+void Rdistances(const float *points, const int *numPoints, const int *dim,
+                float *distances, const char **method, const float *p);
+
+/* Replaces Rdistclust (src/rinterface.cpp:172-204, decl src/rinterface.h:50-52).
+ * The distance stage runs (device-full layout, p = 2 as at
+ * src/rinterface.cpp:182); the hierarchical-clustering stage
+ * (src/hcluster.cpp) is not part of this library, so the call then fails
+ * with GT_ERR_UNSUPPORTED and leaves merge/order/val untouched. */
+void Rdistclust(const char **distmethod, const char **clustmethod,
+                const float *points, const int *numPoints, const int *dim,
+                int *merge, int *order, float *val);
+
+/* Replace cuCompile / unloadPackage (src/cudaUtils.cpp:117-165, 167-195),
+ * called from R/zzz.R:34-38 and :41-43.  Kernels are precompiled gfx950 code
+ * objects, so both only record that the package was loaded/unloaded. */
+void cuCompile(const int *numFiles, const char **cuFilenames, const char **cuSrc);
+void unloadPackage(void);
+
+/* Replace rsetDevice / rgetDevice (src/rinterface.cpp:79-85, bodies
+ * src/correlation.cpp:163-175): range-checked hipSetDevice / hipGetDevice. */
+void rsetDevice(const int *device);
+void rgetDevice(int *device);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Native API                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *gt_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int gt_version(void);
+
+/* getDistEnum (src/rinterface.cpp:147-156): exact name -> method, anything
+ * else -> GT_EUCLIDEAN. */
+int gt_method_from_name(const char *name);
+
+/* Bytes of device workspace gt_dist_ws needs for this request (0 if none). */
+int gt_dist_workspace_size(int64_t n, int64_t k, int method, int precision,
+                           size_t *bytes);
+
+/* Device-resident all-pairs distances of the rows of X.  The replacement of
+ * distance_device (src/distance.cpp:209-257) with a packed or full output.
+ * Workspace, when the request needs one, is taken from a stream-ordered pool
+ * (hipMallocAsync / hipFreeAsync on 'stream'). */
+int gt_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+            int precision, int layout, float *out, int64_t ldd, void *stream);
+
+/* Same, with a caller-provided device workspace of gt_dist_workspace_size
+ * bytes (no allocation inside). */
+int gt_dist_ws(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+               int precision, int layout, float *out, int64_t ldd,
+               void *workspace, size_t workspace_bytes, void *stream);
+
+/* Cross distances between the rows of A (na x k) and B (nb x k), CROSS
+ * layout.  The semantics of the unreachable *_kernel cross kernels
+ * (inst/cuda/distance.cu:3-38, 80-114, 156-190, 234-271, 318-364, 417-451),
+ * FP32 path, with R's canberra rule. */
+int gt_dist_cross(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
+                  int64_t ldb, int64_t k, int method, float p, float *out, int64_t ldd,
+                  void *stream);
+
+/* Host-to-host call: the counterpart of distance() (src/distance.cpp:259-316)
+ * with either output layout; synchronous. */
+int gt_dist_host(const float *x_host, int64_t n, int64_t k, int method, float p,
+                 int precision, int layout, float *out_host);
+
+/* distanceLeaveOnGpu (src/distance.cpp:318-344): copies the host points to the
+ * device, computes the FULL layout into a new device buffer whose ownership
+ * passes to the caller (*d_out, row pitch in bytes in *pitch_bytes); release
+ * it with gt_free. */
+int gt_dist_device_full(const float *x_host, int64_t n, int64_t k, int method, float p,
+                        float **d_out, size_t *pitch_bytes);
+
+int gt_free(void *device_ptr);
+
+/* Backward of the packed distance vector: dX (n x k, row stride lddx) =
+ * sum_j G_ij * d(d_ij)/d(x_i) for euclidean (any precision flag is ignored:
+ * FP32) and minkowski (p >= 1).  G is packed dL/dD.  D is recomputed
+ * tile-by-tile and never stored.  Pairs with d_ij == 0 contribute 0. */
+int gt_dist_backward(const float *x, int64_t n, int64_t k, int64_t ldx, int method,
+                     float p, const float *g_packed, float *dx, int64_t lddx,
+                     void *stream);
+
+/* Fused metric-MDS stress S = sum_{i<j} (||y_i - y_j|| - delta_ij)^2 with
+ * delta packed.  Writes dS/dY (n x q, stride lddy) when dy != NULL and the
+ * stress into *stress_dev (one double on the device; FP64 is used only for
+ * this scalar reduction).  D is recomputed per tile, never stored. */
+int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy,
+                  const float *delta_packed, float *dy, int64_t lddy,
+                  double *stress_dev, void *stream);
+
+/* Batched distances over many independent point sets in one persistent
+ * launch.  Set s has sizes[s] rows of dims[s] features starting at element
+ * x_offsets[s] of x (row stride dims[s]); its packed output starts at element
+ * out_offsets[s] of out.  All descriptor arrays are device arrays of nsets
+ * int64.  Supported methods: euclidean, manhattan, maximum (FP32 path).
+ * 'tile' selects the tile shape (0 = default; see gt_batched_tile_count). */
+int gt_dist_batched(const float *x, const int64_t *x_offsets, const int64_t *sizes,
+                    const int64_t *dims, int64_t nsets, float *out,
+                    const int64_t *out_offsets, int method, int tile,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* Workspace bytes gt_dist_batched needs (tile schedule), given the host copy
+ * of sizes/dims. */
+int gt_dist_batched_workspace_size(const int64_t *sizes_host, const int64_t *dims_host,
+                                   int64_t nsets, int tile, size_t *bytes);
+
+/* Number of tile shapes gt_dist_batched offers (ids 1..count). */
+int gt_batched_tile_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUTOOLS_AMD_H */

@@ -1,0 +1,4 @@
+"""FP64 CPU oracle of the pairwise-distance path -- test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import it.
+"""

@@ -1,0 +1,151 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/dist_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the cpu_baseline leg of ``bench.py``.  The product package never imports it.
+
+The oracle restates R ``stats::dist`` in FP64 for the six methods of gputools'
+``gpuDist`` (R/gpuHclust.R:1-39, kernels inst/cuda/distance.cu:40-493); see the
+header of dist_oracle.c for the per-method citations and where R and gputools
+differ.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+METHODS = ("euclidean", "maximum", "manhattan", "canberra", "binary", "minkowski")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+
+
+def build() -> None:
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_packed_index.argtypes = [_i64, _i64, _i64]
+        L.oracle_packed_index.restype = _i64
+        L.oracle_dist_packed.argtypes = [_f32p, _i64, _i64, _i64, ctypes.c_int,
+                                         ctypes.c_double, _f64p, ctypes.c_int]
+        L.oracle_dist_packed.restype = ctypes.c_int
+        L.oracle_dist_pairs.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64,
+                                        ctypes.c_int, ctypes.c_double, _f64p, ctypes.c_int]
+        L.oracle_dist_pairs.restype = ctypes.c_int
+        L.oracle_dist_cross.argtypes = [_f32p, _i64, _f32p, _i64, _i64, ctypes.c_int,
+                                        ctypes.c_double, _f64p, ctypes.c_int]
+        L.oracle_dist_cross.restype = ctypes.c_int
+        L.oracle_compare_packed.argtypes = [_f32p, _i64, _i64, _i64, ctypes.c_int,
+                                            ctypes.c_double, _f32p, ctypes.c_double,
+                                            ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                            ctypes.c_int]
+        L.oracle_compare_packed.restype = ctypes.c_double
+        L.oracle_dist_backward.argtypes = [_f32p, _i64, _i64, ctypes.c_int, ctypes.c_double,
+                                           _f64p, _f64p, ctypes.c_int]
+        L.oracle_dist_backward.restype = ctypes.c_int
+        L.oracle_mds_stress.argtypes = [_f32p, _i64, _i64, _f64p, ctypes.c_void_p,
+                                        ctypes.c_int]
+        L.oracle_mds_stress.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def method_id(method) -> int:
+    if isinstance(method, str):
+        return METHODS.index(method)
+    return int(method)
+
+
+def _f32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+
+
+def dist(x, method="euclidean", p=2.0, nthreads=0) -> np.ndarray:
+    """Packed FP64 distances of the rows of x, in R 'dist' order."""
+    x = _f32(x)
+    if x.ndim != 2:
+        raise ValueError("x must be 2-D")
+    n, k = x.shape
+    out = np.empty(n * (n - 1) // 2, dtype=np.float64)
+    rc = lib().oracle_dist_packed(x, n, k, k, method_id(method), float(p), out, nthreads)
+    if rc != 0:
+        raise ValueError("oracle_dist_packed rejected its arguments")
+    return out
+
+
+def dist_pairs(x, ii, jj, method="euclidean", p=2.0, nthreads=0) -> np.ndarray:
+    """FP64 distance of the listed row pairs (ii[q], jj[q])."""
+    x = _f32(x)
+    n, k = x.shape
+    ii = np.ascontiguousarray(ii, dtype=np.int64)
+    jj = np.ascontiguousarray(jj, dtype=np.int64)
+    out = np.empty(ii.shape[0], dtype=np.float64)
+    rc = lib().oracle_dist_pairs(x, k, k, ii, jj, ii.shape[0], method_id(method), float(p),
+                                 out, nthreads)
+    if rc != 0:
+        raise ValueError("oracle_dist_pairs rejected its arguments")
+    return out
+
+
+def dist_cross(a, b, method="euclidean", p=2.0, nthreads=0) -> np.ndarray:
+    """FP64 cross distances, nb x na row-major (out[y, x] = d(a[x], b[y]))."""
+    a, b = _f32(a), _f32(b)
+    na, k = a.shape
+    nb = b.shape[0]
+    out = np.empty(na * nb, dtype=np.float64)
+    rc = lib().oracle_dist_cross(a, na, b, nb, k, method_id(method), float(p), out, nthreads)
+    if rc != 0:
+        raise ValueError("oracle_dist_cross rejected its arguments")
+    return out.reshape(nb, na)
+
+
+def compare_packed(x, got, method="euclidean", p=2.0, abs_floor=1e-30, nthreads=0):
+    """Max relative error of an FP32 packed result vs the oracle (streamed)."""
+    x = _f32(x)
+    n, k = x.shape
+    got = np.ascontiguousarray(got, dtype=np.float32)
+    wi, wj = _i64(-1), _i64(-1)
+    err = lib().oracle_compare_packed(x, n, k, k, method_id(method), float(p), got,
+                                      float(abs_floor), ctypes.byref(wi), ctypes.byref(wj),
+                                      nthreads)
+    return err, (wi.value, wj.value)
+
+
+def dist_backward(x, g_packed, method="euclidean", p=2.0, nthreads=0) -> np.ndarray:
+    x = _f32(x)
+    n, k = x.shape
+    g = np.ascontiguousarray(g_packed, dtype=np.float64)
+    dx = np.empty((n, k), dtype=np.float64)
+    rc = lib().oracle_dist_backward(x, n, k, method_id(method), float(p), g, dx, nthreads)
+    if rc != 0:
+        raise ValueError("oracle_dist_backward rejected its arguments")
+    return dx
+
+
+def mds_stress(y, delta, with_grad=True, nthreads=0):
+    y = _f32(y)
+    n, q = y.shape
+    delta = np.ascontiguousarray(delta, dtype=np.float64)
+    dy = np.empty((n, q), dtype=np.float64) if with_grad else None
+    s = lib().oracle_mds_stress(y, n, q, delta,
+                                dy.ctypes.data if dy is not None else None, nthreads)
+    return (s, dy) if with_grad else s
+
+
+def packed_index(n, i, j) -> int:
+    return int(lib().oracle_packed_index(n, i, j))
