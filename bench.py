@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--k", type=int, default=N_FEATURES)
     ap.add_argument("--fp32-steps", type=int, default=None,
                     help="timed steps of the plain FP32 comparison (default: --steps)")
-    ap.add_argument("--sample-rows", type=int, default=4096,
+    ap.add_argument("--sample-rows", type=int, default=11586,
                     help="rows whose all pairs form the CPU-baseline / error sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
