@@ -96,7 +96,7 @@ int workspace_bytes(int64_t n, int64_t k, int method, int precision, size_t *byt
         if (method == GT_BINARY)
             b = round_up((size_t)n * (size_t)gt::binary_words(k) * sizeof(uint32_t), 256);
         else if (precision != GT_PREC_FP32)
-            b = gt::gram_workspace_bytes(n, k);
+            b = gt::gram_workspace_bytes(n, k, precision);
     }
     *bytes = b;
     return GT_OK;

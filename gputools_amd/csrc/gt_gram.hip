@@ -1,21 +1,606 @@
-// gt_gram.hip -- split-precision MFMA Gram path for the euclidean distance.
+// gt_gram.hip -- split-precision MFMA Gram form of the euclidean distance.
+//
+// d(x_i, x_j)^2 = ||x_i||^2 + ||x_j||^2 - 2 x_i.x_j.  The dot products of a
+// 128 x 128 tile of pairs run on the gfx950 matrix cores
+// (v_mfma_f32_32x32x16_{bf16,f16}) with FP32 accumulation.  A pre-pass splits
+// every FP32 input value once into 16-bit terms (stored in the workspace), and
+// the tile kernel accumulates the products whose error terms matter:
+//   BF16 (8-bit significand): three terms x = h + m + l, exact (x - h and
+//     x - h - m are exact in FP32 and the last remainder has at most 8
+//     significant bits); six products hh' + hm' + mh' + hl' + lh' + mm'; the
+//     dropped ml' + lm' + ll' are below 2^-23 |x_c y_c| per feature.
+//   FP16 (11-bit significand): two terms x = h + l (+ r, |r| <= 2^-22|x|),
+//     three products hh' + hl' + lh'; dropped terms below 3 * 2^-22 |x_c y_c|.
+//     Rows are first scaled by a power of two so that max |x| lies in
+//     [2^14, 2^15) (keeps l out of the subnormal range and h below the FP16
+//     maximum); the scale is undone exactly on the dot product.
+// (A two-term BF16 split leaves ~3 * 2^-16 |x_c y_c| per feature, which no
+// cancellation threshold can bring under 1e-5 for small k.)
+//
+// The Gram form loses relative accuracy when d^2 is small against
+// ||x||^2 + ||y||^2 (cancellation).  Every pair with
+//   d^2 < tau (||x||^2 + ||y||^2)
+// is flagged in the epilogue and recomputed in the same workgroup from exact
+// FP32 differences, sum (x_c - y_c)^2, with the FP32 rows re-streamed through
+// LDS in 32-feature chunks (two-level summation).  Flagged pairs are processed
+// through an LDS list of 1024 entries per pass, as many passes as needed, so
+// any number of near-coincident pairs is handled.  tau is derived on the host
+// from k (gram_tau) so that the Gram-form error stays below 1.6e-5 d^2.
+//
+// The reference computes this distance with exact differences only
+// (inst/cuda/distance.cu:40-78); this path produces the same distance within
+// the stated tolerance (max relative error <= 1e-5 against FP64).
 #include "gt_common.h"
 #include "gt_internal.h"
 
 namespace gt {
+namespace {
 
-size_t gram_workspace_bytes(int64_t n, int64_t k)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+enum { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
+
+constexpr int GBM = 128;        // tile side (points)
+constexpr int GBK = 32;         // features per main-loop chunk (terms are padded to it)
+constexpr int GLDK = GBK + 8;   // LDS row length, 16-bit elements (80 B rows)
+constexpr int GTHREADS = 256;   // 4 waves, each a 64 x 64 quadrant (2 x 2 MFMA blocks)
+constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
+constexpr int FBK = 32;         // features per fix-up chunk
+constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
+constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
+constexpr int FPER = FCAP / GTHREADS;
+constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
+
+constexpr int TERM_BYTES = GBM * GLDK * 2;   // one split term of one side
+constexpr int FIX_BYTES = 2 * GBM * FLD * 4; // fix-up staging, both sides
+constexpr int LIST_BYTES = FCAP * 4;
+
+// LDS: [main-loop split terms | (aliased) fix-up staging + pair list] [norms, scales, scan]
+template <int NT>
+struct GramLds {
+    static constexpr int MAIN = 2 * NT * TERM_BYTES;
+    static constexpr int FIX = FIX_BYTES + LIST_BYTES;
+    static constexpr int REGION = MAIN > FIX ? MAIN : FIX;
+    static constexpr int BYTES = REGION + 4 * GBM * 4 + 16 * 4;
+};
+
+struct GramArgs {
+    const float *x;        // FP32 points (fix-up)
+    int64_t n, k, ldx;
+    const unsigned short *terms;  // NT x n x kp split terms
+    int64_t kp;            // k padded to GBK
+    const float *norms;    // ||x_i||^2 (FP32)
+    const float *iscale;   // FP16 path: inverse power-of-two row scale
+    float *out;
+    int64_t ldd;
+    int64_t T;             // tiles per side
+    int64_t id_offset, ids_this;
+    float tau;
+};
+
+template <int SPLIT>
+struct Split;
+
+template <>
+struct Split<SPLIT_BF16> {
+    typedef bf16x8_t vec;
+    static constexpr int NT = 3;      // split terms
+    static __device__ __forceinline__ void split(float v, unsigned short *t)
+    {
+        const __bf16 h = (__bf16)v;
+        const float r1 = v - (float)h;     // exact
+        const __bf16 m = (__bf16)r1;
+        const __bf16 l = (__bf16)(r1 - (float)m);   // r1 - m exact, and exact in BF16
+        t[0] = __builtin_bit_cast(unsigned short, h);
+        t[1] = __builtin_bit_cast(unsigned short, m);
+        t[2] = __builtin_bit_cast(unsigned short, l);
+    }
+    static __device__ __forceinline__ f32x16_t mfma(vec a, vec b, f32x16_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Split<SPLIT_FP16> {
+    typedef f16x8_t vec;
+    static constexpr int NT = 2;
+    static __device__ __forceinline__ void split(float v, unsigned short *t)
+    {
+        const _Float16 h = (_Float16)v;
+        const _Float16 l = (_Float16)(v - (float)h);
+        t[0] = __builtin_bit_cast(unsigned short, h);
+        t[1] = __builtin_bit_cast(unsigned short, l);
+    }
+    static __device__ __forceinline__ f32x16_t mfma(vec a, vec b, f32x16_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Pre-pass: split every value once.  Term t of row r, feature c, at
+// terms[(t * n + r) * kp + c]; features k..kp-1 are zero.  One thread per 8
+// consecutive features (one 16 B store per term).
+template <int SPLIT>
+__global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x, int64_t n,
+                                                    int64_t k, int64_t ldx, int64_t kp,
+                                                    const float *__restrict__ scales,
+                                                    unsigned short *__restrict__ terms)
 {
-    (void)k;
-    return ((size_t)n * sizeof(float) + 255) / 256 * 256;
+    constexpr int NT = Split<SPLIT>::NT;
+    const int64_t per_row = kp / 8;
+    const int64_t units = n * per_row;
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < units;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = u / per_row, c0 = (u % per_row) * 8;
+        const float sc = (SPLIT == SPLIT_FP16) ? scales[r] : 1.0f;
+        unsigned short t[8][NT];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float v = (c0 + c < k) ? x[r * ldx + c0 + c] : 0.0f;
+            if (SPLIT == SPLIT_FP16) v *= sc;
+            Split<SPLIT>::split(v, t[c]);
+        }
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            uint4 w;
+            w.x = t[0][q] | ((unsigned)t[1][q] << 16);
+            w.y = t[2][q] | ((unsigned)t[3][q] << 16);
+            w.z = t[4][q] | ((unsigned)t[5][q] << 16);
+            w.w = t[6][q] | ((unsigned)t[7][q] << 16);
+            *reinterpret_cast<uint4 *>(terms + ((int64_t)q * n + r) * kp + c0) = w;
+        }
+    }
+}
+
+// Exclusive scan of one int per thread over the 256-thread block.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) tmp[wave] = incl;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < GTHREADS / 64; ++w) {
+        const int c = tmp[w];
+        if (w < wave) before += c;
+        total += c;
+    }
+    __syncthreads();
+    return before + incl - v;
+}
+
+template <int SPLIT, int LAYOUT, bool VEC4>
+__global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
+{
+    typedef typename Split<SPLIT>::vec vec;
+    constexpr int NT = Split<SPLIT>::NT;
+    typedef GramLds<NT> L;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
+    // split term t of side s (0 = A, rows j; 1 = B, rows i)
+    unsigned short *lterms = reinterpret_cast<unsigned short *>(smem);
+    auto tS = [&](int s, int t) { return lterms + (s * NT + t) * GBM * GLDK; };
+    float *fS = reinterpret_cast<float *>(smem);          // fix-up staging (aliases the terms)
+    unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
+    float *nA = reinterpret_cast<float *>(smem + L::REGION);  // row norms, A side (j)
+    float *nB = nA + GBM;                                 // B side (i)
+    float *iA = nB + GBM;                                 // inverse row scales (FP16 path)
+    float *iB = iA + GBM;
+    int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
+    // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
+    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int64_t st = id / (GSUPER * GSUPER);
+    const int w = (int)(id % (GSUPER * GSUPER));
+    int SI, SJ;
+    tri_tile(st, SI, SJ);
+    const int64_t I = (int64_t)SI * GSUPER + w / GSUPER;
+    const int64_t J = (int64_t)SJ * GSUPER + w % GSUPER;
+    if (I >= args.T || J > I) return;
+    const int64_t i0 = I * GBM, j0 = J * GBM;
+    const int64_t n = args.n, k = args.k, kp = args.kp;
+
+    if (tid < GBM) {
+        const int64_t r = j0 + tid;
+        nA[tid] = r < n ? args.norms[r] : 0.0f;
+        if (SPLIT == SPLIT_FP16) iA[tid] = r < n ? args.iscale[r] : 1.0f;
+    } else {
+        const int64_t r = i0 + (tid - GBM);
+        nB[tid - GBM] = r < n ? args.norms[r] : 0.0f;
+        if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < n ? args.iscale[r] : 1.0f;
+    }
+
+    // ---- main loop: 16 B term loads -> LDS, then the MFMA products ---------
+    // staging unit u = tid + 256 q (q < 2): row u >> 2, features (u & 3) * 8..+7,
+    // for each side and term.
+    uint4 pre[2][NT][2];
+    const unsigned short *gsrc[2][2];   // [side][q] row base pointers
+    bool gok[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int u = tid + GTHREADS * q;
+            const int64_t r = (s ? i0 : j0) + (u >> 2);
+            gok[s][q] = r < n;
+            gsrc[s][q] = args.terms + (gok[s][q] ? r : 0) * kp + (u & 3) * 8;
+        }
+    auto gload = [&](int64_t kb) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                    pre[s][t][q] = gok[s][q] ? *reinterpret_cast<const uint4 *>(
+                                                   gsrc[s][q] + (int64_t)t * n * kp + kb)
+                                             : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int u = tid + GTHREADS * q;
+                    *reinterpret_cast<uint4 *>(tS(s, t) + (u >> 2) * GLDK + (u & 3) * 8) =
+                        pre[s][t][q];
+                }
+    };
+
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+    gload(0);
+    for (int64_t kb = 0; kb < kp; kb += GBK) {
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (kb + GBK < kp) gload(kb + GBK);
+#pragma unroll
+        for (int ks = 0; ks < GBK / 16; ++ks) {
+            const int koff = ks * 16 + (lane >> 5) * 8;
+            vec av[NT][2], bv[NT][2];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    av[t][h] = *reinterpret_cast<const vec *>(
+                        tS(0, t) + (wr * 64 + h * 32 + (lane & 31)) * GLDK + koff);
+                    bv[t][h] = *reinterpret_cast<const vec *>(
+                        tS(1, t) + (wc * 64 + h * 32 + (lane & 31)) * GLDK + koff);
+                }
+            // hh' first, then the cross terms in decreasing magnitude
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[0][b], acc[a][b]);
+                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[1][b], acc[a][b]);
+                    acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[0][b], acc[a][b]);
+                    if (NT == 3) {
+                        acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[NT - 1][b], acc[a][b]);
+                        acc[a][b] = Split<SPLIT>::mfma(av[NT - 1][a], bv[0][b], acc[a][b]);
+                        acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[1][b], acc[a][b]);
+                    }
+                }
+        }
+    }
+
+    // ---- epilogue: Gram-form distance, or flag for exact recompute ---------
+    // 32x32 accumulator layout: element r of lane l is row 8(r/4) + 4(l/32) +
+    // r%4 (A side, j) and column l%32 (B side, i); lanes 0..31 hold 32
+    // consecutive i of one j, so packed stores are contiguous runs.
+    unsigned long long fmask = 0ull;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int64_t j = j0 + lj;
+            const float nj = nA[lj];
+            const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
+            const int64_t rowbase = (LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int li = wc * 64 + b * 32 + (lane & 31);
+                const int64_t i = i0 + li;
+                float g = acc[a][b][r];
+                // undo the row scales one at a time (their product can overflow)
+                if (SPLIT == SPLIT_FP16) g = (g * sj) * iB[li];
+                const float ss = nj + nB[li];
+                const float d2 = fmaf(-2.0f, g, ss);
+                const bool flag = !(d2 >= args.tau * ss);
+                if (i < n && i > j) {
+                    if (!flag) {
+                        const float v = sqrtf(fmaxf(d2, 0.0f));
+                        args.out[rowbase + i] = v;
+                        if (LAYOUT == OUT_FULL) args.out[i * args.ldd + j] = v;
+                    } else {
+                        fmask |= 1ull << ((a * 2 + b) * 16 + r);
+                    }
+                }
+                if (LAYOUT == OUT_FULL && i == j && i < n) args.out[rowbase + i] = 0.0f;
+            }
+        }
+
+    // ---- exact FP32 recompute of the flagged pairs -------------------------
+    int total;
+    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
+    if (total == 0) return;
+    const int64_t ldx = args.ldx;
+    // staging unit e = tid + 256 q: side e >> 10, row (e >> 3) & 127, features (e & 7) * 4
+    float4 fpre[FUNITS];
+    auto fload = [&](int64_t fb) {
+#pragma unroll
+        for (int q = 0; q < FUNITS; ++q) {
+            const int e = tid + GTHREADS * q;
+            const int64_t g = ((e >> 10) ? i0 : j0) + ((e >> 3) & 127);
+            const int64_t kk = fb + (e & 7) * 4;
+            if (VEC4) {
+                fpre[q] = (g < n && kk < k) ? *reinterpret_cast<const float4 *>(args.x + g * ldx + kk)
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                float v[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    v[c] = (g < n && kk + c < k) ? args.x[g * ldx + kk + c] : 0.0f;
+                fpre[q] = make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+    };
+    for (int base = 0; base < total; base += FCAP) {
+        {
+            int rank = myoff;
+            unsigned long long m = fmask;
+            while (m) {
+                const int bit = __builtin_ctzll(m);
+                m &= m - 1;
+                if (rank >= base && rank < base + FCAP) {
+                    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
+                    const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int li = wc * 64 + b * 32 + (lane & 31);
+                    list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
+                }
+                ++rank;
+            }
+        }
+        const int cnt = (total - base) < FCAP ? (total - base) : FCAP;
+        float sum[FPER];
+        unsigned ent[FPER];
+#pragma unroll
+        for (int c = 0; c < FPER; ++c) sum[c] = 0.0f;
+        fload(0);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < FPER; ++c) {
+            const int e = tid + GTHREADS * c;
+            ent[c] = e < cnt ? list[e] : 0u;
+        }
+        for (int64_t fb = 0; fb < k; fb += FBK) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < FUNITS; ++q) {
+                const int e = tid + GTHREADS * q;
+                *reinterpret_cast<float4 *>(fS + ((e >> 10) * GBM + ((e >> 3) & 127)) * FLD +
+                                            (e & 7) * 4) = fpre[q];
+            }
+            __syncthreads();
+            if (fb + FBK < k) fload(fb + FBK);
+#pragma unroll
+            for (int c = 0; c < FPER; ++c) {
+                if (tid + GTHREADS * c < cnt) {
+                    const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD);
+                    const float4 *rb =
+                        reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD);
+                    float part = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < FBK / 4; ++q) {
+                        const float4 u = ra[q], v = rb[q];
+                        float t = u.x - v.x;
+                        part = fmaf(t, t, part);
+                        t = u.y - v.y;
+                        part = fmaf(t, t, part);
+                        t = u.z - v.z;
+                        part = fmaf(t, t, part);
+                        t = u.w - v.w;
+                        part = fmaf(t, t, part);
+                    }
+                    sum[c] += part;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < FPER; ++c) {
+            if (tid + GTHREADS * c < cnt) {
+                const int64_t j = j0 + (ent[c] >> 16), i = i0 + (ent[c] & 0xffffu);
+                const float v = sqrtf(sum[c]);
+                if (LAYOUT == OUT_PACKED) {
+                    args.out[packed_base(n, j) + i] = v;
+                } else {
+                    args.out[j * args.ldd + i] = v;
+                    args.out[i * args.ldd + j] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ||x_i||^2 per row (and, for FP16, the power-of-two row scale and its
+// inverse): one wave per row.
+__global__ __launch_bounds__(256) void row_prep_kernel(const float *__restrict__ x, int64_t n,
+                                                       int64_t k, int64_t ldx,
+                                                       float *__restrict__ norms,
+                                                       float *__restrict__ scales,
+                                                       float *__restrict__ iscales)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < n; row += (int64_t)gridDim.x * 4) {
+        const float *xr = x + row * ldx;
+        float s = 0.0f, m = 0.0f;
+        for (int64_t c = lane; c < k; c += 64) {
+            const float v = xr[c];
+            s = fmaf(v, v, s);
+            m = fmaxf(m, fabsf(v));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            s += __shfl_xor(s, d, 64);
+            m = fmaxf(m, __shfl_xor(m, d, 64));
+        }
+        if (lane == 0) {
+            norms[row] = s;
+            if (scales) {
+                // power of two that brings max|x| into [2^14, 2^15)
+                int e = 0;
+                if (m > 0.0f) {
+                    e = ilogbf(m);
+                    if (e < -100) e = -100;
+                    if (e > 100) e = 100;
+                }
+                scales[row] = ldexpf(1.0f, 14 - e);
+                iscales[row] = ldexpf(1.0f, e - 14);
+            }
+        }
+    }
+}
+
+template <int SPLIT, int LAYOUT, bool VEC4>
+hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
+{
+    const int64_t max_ids = int64_t(1) << 23;
+    for (int64_t off = 0; off < ids; off += max_ids) {
+        args.id_offset = off;
+        args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
+        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4>), dim3((unsigned)args.ids_this),
+                           dim3(GTHREADS), 0, stream, args);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <int SPLIT>
+hipError_t launch_split(int layout, bool vec4, GramArgs &args, const float *scales, int64_t ids,
+                        hipStream_t stream)
+{
+    {
+        const int64_t units = args.n * (args.kp / 8);
+        int64_t blocks = (units + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(split_kernel<SPLIT>, dim3((unsigned)blocks), dim3(256), 0, stream, args.x,
+                           args.n, args.k, args.ldx, args.kp, scales,
+                           const_cast<unsigned short *>(args.terms));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (layout == OUT_PACKED)
+        return vec4 ? launch_tiles<SPLIT, OUT_PACKED, true>(args, ids, stream)
+                    : launch_tiles<SPLIT, OUT_PACKED, false>(args, ids, stream);
+    return vec4 ? launch_tiles<SPLIT, OUT_FULL, true>(args, ids, stream)
+                : launch_tiles<SPLIT, OUT_FULL, false>(args, ids, stream);
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+int64_t padded_k(int64_t k) { return (k + GBK - 1) / GBK * GBK; }
+
+int split_terms(int precision) { return precision == 1 ? 3 : 2; }
+
+}  // namespace
+
+// Workspace: split terms (NT x n x kp 16-bit values), then norms, scales and
+// inverse scales (n floats each).
+size_t gram_workspace_bytes(int64_t n, int64_t k, int precision)
+{
+    const size_t terms = align256((size_t)split_terms(precision) * (size_t)n *
+                                  (size_t)padded_k(k) * sizeof(unsigned short));
+    return terms + 3 * align256((size_t)n * sizeof(float));
+}
+
+// Cancellation threshold: pairs with d^2 < tau (||x||^2 + ||y||^2) are
+// recomputed from exact differences.  tau bounds the Gram-form error of d^2,
+// in units of ||x||^2 + ||y||^2 (u = 2^-24):
+//   split    BF16 three terms: 4u; FP16 two terms: 12u (= 3 * 2^-22)
+//            (|dg| <= c sum|x_c y_c| <= c (||x||^2+||y||^2)/2 and |d(d^2)| = 2|dg|)
+//   accumulation of the MFMA partial sums, one FP32 rounding per product
+//            instruction: statistical bound 4 sqrt(#adds) u
+//   norms    (ceil(k/64) + 6) u for the two row norms together
+//   final    3u for ||x||^2 + ||y||^2 - 2g
+// and divides it by 1.6e-5, the allowed relative error of d^2 (8e-6 on d).
+float gram_tau(int precision, int64_t k)
+{
+    const double u = 1.0 / 16777216.0;
+    const bool bf16 = (precision == 1);
+    const double split = bf16 ? 4.0 * u : 12.0 * u;
+    const double nadds = (bf16 ? 6.0 : 3.0) * (double)((k + 15) / 16);
+    const double accum = 4.0 * sqrt(nadds) * u;
+    const double norms = ((double)((k + 63) / 64) + 6.0) * u;
+    const double total = split + accum + norms + 3.0 * u;
+    return (float)(total / 1.6e-5);
 }
 
 hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int64_t k,
                        int64_t ldx, float *out, int64_t ldd, void *workspace, hipStream_t stream)
 {
-    (void)precision; (void)layout; (void)x; (void)n; (void)k; (void)ldx; (void)out; (void)ldd;
-    (void)workspace; (void)stream;
-    return hipErrorNotSupported;
+    if (n < 2 && layout == OUT_PACKED) return hipSuccess;
+    const int64_t kp = padded_k(k);
+    char *ws = static_cast<char *>(workspace);
+    unsigned short *terms = reinterpret_cast<unsigned short *>(ws);
+    ws += align256((size_t)split_terms(precision) * (size_t)n * (size_t)kp * sizeof(unsigned short));
+    float *norms = reinterpret_cast<float *>(ws);
+    ws += align256((size_t)n * sizeof(float));
+    float *scales = reinterpret_cast<float *>(ws);
+    ws += align256((size_t)n * sizeof(float));
+    float *iscales = reinterpret_cast<float *>(ws);
+    const bool fp16 = (precision == 2);
+    {
+        int64_t blocks = (n + 3) / 4;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, k,
+                           ldx, norms, fp16 ? scales : nullptr, fp16 ? iscales : nullptr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    GramArgs args;
+    args.x = x;
+    args.n = n;
+    args.k = k;
+    args.ldx = ldx;
+    args.terms = terms;
+    args.kp = kp;
+    args.norms = norms;
+    args.iscale = iscales;
+    args.out = out;
+    args.ldd = ldd;
+    args.T = (n + GBM - 1) / GBM;
+    args.tau = gram_tau(precision, k);
+    const int64_t S = (args.T + GSUPER - 1) / GSUPER;
+    const int64_t ids = S * (S + 1) / 2 * GSUPER * GSUPER;
+    const bool vec4 = (k % 4 == 0) && (ldx % 4 == 0) &&
+                      ((reinterpret_cast<uintptr_t>(x) & 15u) == 0);
+    return fp16 ? launch_split<SPLIT_FP16>(layout, vec4, args, scales, ids, stream)
+                : launch_split<SPLIT_BF16>(layout, vec4, args, scales, ids, stream);
 }
 
 }  // namespace gt

@@ -43,8 +43,9 @@ hipError_t launch_fill(float *out, int64_t count, float value, hipStream_t strea
 hipError_t launch_fill_full(float *out, int64_t n, int64_t ldd, float offdiag,
                             hipStream_t stream);
 
-// Split-precision MFMA Gram path (euclidean).  Workspace: row norms (n floats).
-size_t gram_workspace_bytes(int64_t n, int64_t k);
+// Split-precision MFMA Gram path (euclidean).  Workspace: split terms, row
+// norms and row scales (gram_workspace_bytes).
+size_t gram_workspace_bytes(int64_t n, int64_t k, int precision);
 hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int64_t k,
                        int64_t ldx, float *out, int64_t ldd, void *workspace,
                        hipStream_t stream);

@@ -48,9 +48,13 @@ enum gt_method {
 };
 
 /* Arithmetic of the euclidean path.  FP32 = exact differences on the VALU
- * (every method).  BF16X3 / FP16X3 = Gram form ||x||^2 + ||y||^2 - 2 x.y on
- * MFMA with a hi+lo split (three products, FP32 accumulation) and exact FP32
- * recomputation of near-coincident pairs; euclidean only. */
+ * (every method).  The split forms compute the Gram form
+ * ||x||^2 + ||y||^2 - 2 x.y on MFMA with FP32 accumulation and recompute
+ * near-coincident pairs (d^2 < tau (||x||^2 + ||y||^2)) from exact FP32
+ * differences; euclidean only, max relative error <= 1e-5:
+ *   BF16X3  each value split into three BF16 terms, six MFMA products
+ *   FP16X3  each row scaled by a power of two, each value split into two
+ *           FP16 terms (hi + lo), three MFMA products */
 enum gt_precision {
     GT_PREC_FP32 = 0,
     GT_PREC_BF16X3 = 1,
