@@ -1,7 +1,7 @@
 // gt_gram.hip -- split-precision MFMA Gram form of the euclidean distance.
 //
 // d(x_i, x_j)^2 = ||x_i||^2 + ||x_j||^2 - 2 x_i.x_j.  The dot products of a
-// 128 x 128 tile of pairs run on the gfx950 matrix cores
+// 256 x 256 tile of pairs run on the gfx950 matrix cores
 // (v_mfma_f32_32x32x16_{bf16,f16}) with FP32 accumulation.  A pre-pass splits
 // every FP32 input value once into 16-bit terms (stored in the workspace), and
 // the tile kernel accumulates the products whose error terms matter:
@@ -42,25 +42,30 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 enum { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
 
-constexpr int GBM = 128;        // tile side (points)
-constexpr int GBK = 32;         // features per main-loop chunk (terms are padded to it)
-constexpr int GLDK = GBK + 8;   // LDS row length, 16-bit elements (80 B rows)
-constexpr int GTHREADS = 256;   // 4 waves, each a 64 x 64 quadrant (2 x 2 MFMA blocks)
-constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
+constexpr int GBM = 256;        // tile side (points)
+constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
+constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
+constexpr int GLDK = GBK + 8;   // LDS row length, 16-bit elements (48 B rows: b128
+                                // operand reads of 8 consecutive rows hit distinct banks)
+constexpr int GTHREADS = 512;   // 8 waves in a 4 (A) x 2 (B) grid, each 64 x 128 pairs
+constexpr int GSUPER = 2;       // super-tile side in tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
 constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
 constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
 constexpr int FPER = FCAP / GTHREADS;
 constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
+constexpr int GUNITS = GBM * GBK / 8 / GTHREADS;      // 16 B term units per thread, per side and term (1)
 
 constexpr int TERM_BYTES = GBM * GLDK * 2;   // one split term of one side
 constexpr int FIX_BYTES = 2 * GBM * FLD * 4; // fix-up staging, both sides
 constexpr int LIST_BYTES = FCAP * 4;
 
-// LDS: [main-loop split terms | (aliased) fix-up staging + pair list] [norms, scales, scan]
+// LDS: [main-loop split terms, two stages | (aliased) fix-up staging + pair
+// list] [norms, scales, scan]
 template <int NT>
 struct GramLds {
-    static constexpr int MAIN = 2 * NT * TERM_BYTES;
+    static constexpr int STAGE = 2 * NT * TERM_BYTES;
+    static constexpr int MAIN = 2 * STAGE;
     static constexpr int FIX = FIX_BYTES + LIST_BYTES;
     static constexpr int REGION = MAIN > FIX ? MAIN : FIX;
     static constexpr int BYTES = REGION + 4 * GBM * 4 + 16 * 4;
@@ -70,7 +75,7 @@ struct GramArgs {
     const float *x;        // FP32 points (fix-up)
     int64_t n, k, ldx;
     const unsigned short *terms;  // NT x n x kp split terms
-    int64_t kp;            // k padded to GBK
+    int64_t kp;            // k padded to KPAD
     const float *norms;    // ||x_i||^2 (FP32)
     const float *iscale;   // FP16 path: inverse power-of-two row scale
     float *out;
@@ -186,9 +191,9 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     constexpr int NT = Split<SPLIT>::NT;
     typedef GramLds<NT> L;
     __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
-    // split term t of side s (0 = A, rows j; 1 = B, rows i)
+    // split term t of side s (0 = A, rows j; 1 = B, rows i) in stage g
     unsigned short *lterms = reinterpret_cast<unsigned short *>(smem);
-    auto tS = [&](int s, int t) { return lterms + (s * NT + t) * GBM * GLDK; };
+    auto tS = [&](int g, int s, int t) { return lterms + ((g * 2 + s) * NT + t) * GBM * GLDK; };
     float *fS = reinterpret_cast<float *>(smem);          // fix-up staging (aliases the terms)
     unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
     float *nA = reinterpret_cast<float *>(smem + L::REGION);  // row norms, A side (j)
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave >> 1, wc = wave & 1;   // A rows wr*64.., B rows wc*128..
 
     // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
     // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
@@ -223,94 +228,111 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < n ? args.iscale[r] : 1.0f;
     }
 
-    // ---- main loop: 16 B term loads -> LDS, then the MFMA products ---------
-    // staging unit u = tid + 256 q (q < 2): row u >> 2, features (u & 3) * 8..+7,
-    // for each side and term.
-    uint4 pre[2][NT][2];
-    const unsigned short *gsrc[2][2];   // [side][q] row base pointers
-    bool gok[2][2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int u = tid + GTHREADS * q;
-            const int64_t r = (s ? i0 : j0) + (u >> 2);
-            gok[s][q] = r < n;
-            gsrc[s][q] = args.terms + (gok[s][q] ? r : 0) * kp + (u & 3) * 8;
-        }
-    auto gload = [&](int64_t kb) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-                    pre[s][t][q] = gok[s][q] ? *reinterpret_cast<const uint4 *>(
-                                                   gsrc[s][q] + (int64_t)t * n * kp + kb)
-                                             : make_uint4(0u, 0u, 0u, 0u);
-    };
-    auto sstore = [&]() {
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const int u = tid + GTHREADS * q;
-                    *reinterpret_cast<uint4 *>(tS(s, t) + (u >> 2) * GLDK + (u & 3) * 8) =
-                        pre[s][t][q];
-                }
-    };
+    // ---- main loop: 16 B term loads -> LDS (two stages), MFMA products -----
+    // Each thread stages one 16 B unit per side and term: row tid >> 1,
+    // features (tid & 1) * 8..+7 of the chunk.  Rows past n are clamped to
+    // row n-1: they only feed pairs that are never stored.
+    // (plain macros with constant indices keep the staging registers out of
+    // private memory)
+    const int64_t term_stride = n * kp;
+    const unsigned short *gsA, *gsB;
+    {
+        int64_t r = j0 + (tid >> 1);
+        gsA = args.terms + (r < n ? r : n - 1) * kp + (tid & 1) * 8;
+        r = i0 + (tid >> 1);
+        gsB = args.terms + (r < n ? r : n - 1) * kp + (tid & 1) * 8;
+    }
+    const int soff = (tid >> 1) * GLDK + (tid & 1) * 8;
+    uint4 pA0, pA1, pA2, pB0, pB1, pB2;
+#define GT_GLOAD(kb)                                                                       \
+    do {                                                                                   \
+        pA0 = *reinterpret_cast<const uint4 *>(gsA + (kb));                                \
+        pB0 = *reinterpret_cast<const uint4 *>(gsB + (kb));                                \
+        pA1 = *reinterpret_cast<const uint4 *>(gsA + term_stride + (kb));                  \
+        pB1 = *reinterpret_cast<const uint4 *>(gsB + term_stride + (kb));                  \
+        if (NT == 3) {                                                                     \
+            pA2 = *reinterpret_cast<const uint4 *>(gsA + 2 * term_stride + (kb));          \
+            pB2 = *reinterpret_cast<const uint4 *>(gsB + 2 * term_stride + (kb));          \
+        }                                                                                  \
+    } while (0)
+#define GT_SSTORE(g)                                                                       \
+    do {                                                                                   \
+        *reinterpret_cast<uint4 *>(tS(g, 0, 0) + soff) = pA0;                              \
+        *reinterpret_cast<uint4 *>(tS(g, 1, 0) + soff) = pB0;                              \
+        *reinterpret_cast<uint4 *>(tS(g, 0, 1) + soff) = pA1;                              \
+        *reinterpret_cast<uint4 *>(tS(g, 1, 1) + soff) = pB1;                              \
+        if (NT == 3) {                                                                     \
+            *reinterpret_cast<uint4 *>(tS(g, 0, NT - 1) + soff) = pA2;                     \
+            *reinterpret_cast<uint4 *>(tS(g, 1, NT - 1) + soff) = pB2;                     \
+        }                                                                                  \
+    } while (0)
 
-    f32x16_t acc[2][2];
+    f32x16_t acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
-    gload(0);
-    for (int64_t kb = 0; kb < kp; kb += GBK) {
-        __syncthreads();
-        sstore();
-        __syncthreads();
-        if (kb + GBK < kp) gload(kb + GBK);
+    const int nchunks = (int)(kp / GBK);
+    const int koff = (lane >> 5) * 8;
+    GT_GLOAD(0);
+    GT_SSTORE(0);
+    if (nchunks > 1) GT_GLOAD(GBK);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int g = c & 1;
+        vec av[NT][2];
 #pragma unroll
-        for (int ks = 0; ks < GBK / 16; ++ks) {
-            const int koff = ks * 16 + (lane >> 5) * 8;
-            vec av[NT][2], bv[NT][2];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    av[t][h] = *reinterpret_cast<const vec *>(
-                        tS(0, t) + (wr * 64 + h * 32 + (lane & 31)) * GLDK + koff);
-                    bv[t][h] = *reinterpret_cast<const vec *>(
-                        tS(1, t) + (wc * 64 + h * 32 + (lane & 31)) * GLDK + koff);
-                }
-            // hh' first, then the cross terms in decreasing magnitude
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int a = 0; a < 2; ++a)
+                av[t][a] = *reinterpret_cast<const vec *>(
+                    tS(g, 0, t) + (wr * 64 + a * 32 + (lane & 31)) * GLDK + koff);
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[0][b], acc[a][b]);
-                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[1][b], acc[a][b]);
-                    acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[0][b], acc[a][b]);
-                    if (NT == 3) {
-                        acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[NT - 1][b], acc[a][b]);
-                        acc[a][b] = Split<SPLIT>::mfma(av[NT - 1][a], bv[0][b], acc[a][b]);
-                        acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[1][b], acc[a][b]);
-                    }
+        for (int b = 0; b < 4; ++b) {
+            vec bv[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                bv[t] = *reinterpret_cast<const vec *>(
+                    tS(g, 1, t) + (wc * 128 + b * 32 + (lane & 31)) * GLDK + koff);
+            // hh' first, then the cross terms in decreasing magnitude
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[0], acc[a][b]);
+                acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[1], acc[a][b]);
+                acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[0], acc[a][b]);
+                if (NT == 3) {
+                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[NT - 1], acc[a][b]);
+                    acc[a][b] = Split<SPLIT>::mfma(av[NT - 1][a], bv[0], acc[a][b]);
+                    acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[1], acc[a][b]);
                 }
+            }
         }
+        // The other stage was last read in chunk c-1, which every wave
+        // finished before the barrier that ended it.
+        if (c + 1 < nchunks) GT_SSTORE(g ^ 1);
+        __syncthreads();
+        if (c + 2 < nchunks) GT_GLOAD((int64_t)(c + 2) * GBK);
     }
+#undef GT_GLOAD
+#undef GT_SSTORE
 
     // ---- epilogue: Gram-form distance, or flag for exact recompute ---------
     // 32x32 accumulator layout: element r of lane l is row 8(r/4) + 4(l/32) +
     // r%4 (A side, j) and column l%32 (B side, i); lanes 0..31 hold 32
     // consecutive i of one j, so packed stores are contiguous runs.
-    unsigned long long fmask = 0ull;
+    const bool interior = (I > J) && (i0 + GBM <= n);   // every pair valid, i > j
+    const int li0 = wc * 128 + (lane & 31);
+    float ni[4], sb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        ni[b] = nB[li0 + b * 32];
+        sb[b] = (SPLIT == SPLIT_FP16) ? iB[li0 + b * 32] : 1.0f;
+    }
+    const float tau = args.tau;
+    unsigned long long fmask[2] = {0ull, 0ull};
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -319,51 +341,52 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
             const int64_t j = j0 + lj;
             const float nj = nA[lj];
             const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
-            const int64_t rowbase = (LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd;
+            float *rowp = args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd) +
+                          i0 + li0;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int li = wc * 64 + b * 32 + (lane & 31);
-                const int64_t i = i0 + li;
+            for (int b = 0; b < 4; ++b) {
                 float g = acc[a][b][r];
                 // undo the row scales one at a time (their product can overflow)
-                if (SPLIT == SPLIT_FP16) g = (g * sj) * iB[li];
-                const float ss = nj + nB[li];
+                if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
+                const float ss = nj + ni[b];
                 const float d2 = fmaf(-2.0f, g, ss);
-                const bool flag = !(d2 >= args.tau * ss);
-                if (i < n && i > j) {
+                const bool flag = !(d2 >= tau * ss);
+                const int li = li0 + b * 32;
+                const bool valid = interior || (i0 + li < n && i0 + li > j);
+                if (valid) {
                     if (!flag) {
                         const float v = sqrtf(fmaxf(d2, 0.0f));
-                        args.out[rowbase + i] = v;
-                        if (LAYOUT == OUT_FULL) args.out[i * args.ldd + j] = v;
+                        rowp[b * 32] = v;
+                        if (LAYOUT == OUT_FULL) args.out[(i0 + li) * args.ldd + j] = v;
                     } else {
-                        fmask |= 1ull << ((a * 2 + b) * 16 + r);
+                        fmask[a] |= 1ull << (b * 16 + r);
                     }
                 }
-                if (LAYOUT == OUT_FULL && i == j && i < n) args.out[rowbase + i] = 0.0f;
+                if (LAYOUT == OUT_FULL && !interior && i0 + li == j && j < n) rowp[b * 32] = 0.0f;
             }
         }
 
     // ---- exact FP32 recompute of the flagged pairs -------------------------
     int total;
-    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
+    const int myoff = block_exclusive_scan(__popcll(fmask[0]) + __popcll(fmask[1]), scan_tmp, total);
     if (total == 0) return;
     const int64_t ldx = args.ldx;
-    // staging unit e = tid + 256 q: side e >> 10, row (e >> 3) & 127, features (e & 7) * 4
+    // staging unit e = tid + 512 q: side e >> 11, row (e >> 3) & 255, features (e & 7) * 4
     float4 fpre[FUNITS];
     auto fload = [&](int64_t fb) {
 #pragma unroll
         for (int q = 0; q < FUNITS; ++q) {
             const int e = tid + GTHREADS * q;
-            const int64_t g = ((e >> 10) ? i0 : j0) + ((e >> 3) & 127);
+            int64_t g = ((e >> 11) ? i0 : j0) + ((e >> 3) & (GBM - 1));
+            g = g < n ? g : n - 1;
             const int64_t kk = fb + (e & 7) * 4;
             if (VEC4) {
-                fpre[q] = (g < n && kk < k) ? *reinterpret_cast<const float4 *>(args.x + g * ldx + kk)
-                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+                fpre[q] = kk < k ? *reinterpret_cast<const float4 *>(args.x + g * ldx + kk)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
                 float v[4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    v[c] = (g < n && kk + c < k) ? args.x[g * ldx + kk + c] : 0.0f;
+                for (int c = 0; c < 4; ++c) v[c] = (kk + c < k) ? args.x[g * ldx + kk + c] : 0.0f;
                 fpre[q] = make_float4(v[0], v[1], v[2], v[3]);
             }
         }
@@ -371,17 +394,20 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     for (int base = 0; base < total; base += FCAP) {
         {
             int rank = myoff;
-            unsigned long long m = fmask;
-            while (m) {
-                const int bit = __builtin_ctzll(m);
-                m &= m - 1;
-                if (rank >= base && rank < base + FCAP) {
-                    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
-                    const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    const int li = wc * 64 + b * 32 + (lane & 31);
-                    list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                unsigned long long m = fmask[a];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (rank >= base && rank < base + FCAP) {
+                        const int b = bit >> 4, r = bit & 15;
+                        const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        const int li = li0 + b * 32;
+                        list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
+                    }
+                    ++rank;
                 }
-                ++rank;
             }
         }
         const int cnt = (total - base) < FCAP ? (total - base) : FCAP;
@@ -401,7 +427,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
 #pragma unroll
             for (int q = 0; q < FUNITS; ++q) {
                 const int e = tid + GTHREADS * q;
-                *reinterpret_cast<float4 *>(fS + ((e >> 10) * GBM + ((e >> 3) & 127)) * FLD +
+                *reinterpret_cast<float4 *>(fS + ((e >> 11) * GBM + ((e >> 3) & (GBM - 1))) * FLD +
                                             (e & 7) * 4) = fpre[q];
             }
             __syncthreads();
@@ -523,7 +549,7 @@ hipError_t launch_split(int layout, bool vec4, GramArgs &args, const float *scal
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-int64_t padded_k(int64_t k) { return (k + GBK - 1) / GBK * GBK; }
+int64_t padded_k(int64_t k) { return (k + KPAD - 1) / KPAD * KPAD; }
 
 int split_terms(int precision) { return precision == 1 ? 3 : 2; }
 
