@@ -45,8 +45,7 @@ enum { SPLIT_BF16 = 0, SPLIT_FP16 = 1 };
 constexpr int GBM = 256;        // tile side (points)
 constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
-constexpr int GLDK = GBK + 8;   // LDS row length, 16-bit elements (48 B rows: b128
-                                // operand reads of 8 consecutive rows hit distinct banks)
+constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
 constexpr int GTHREADS = 512;   // 8 waves in a 4 (A) x 2 (B) grid, each 64 x 128 pairs
 constexpr int GSUPER = 2;       // super-tile side in tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
@@ -54,18 +53,18 @@ constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned r
 constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
 constexpr int FPER = FCAP / GTHREADS;
 constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
-constexpr int GUNITS = GBM * GBK / 8 / GTHREADS;      // 16 B term units per thread, per side and term (1)
 
-constexpr int TERM_BYTES = GBM * GLDK * 2;   // one split term of one side
+constexpr int TERM_BYTES = GBM * GROWB;      // one split term of one side, one chunk (8 KB)
 constexpr int FIX_BYTES = 2 * GBM * FLD * 4; // fix-up staging, both sides
 constexpr int LIST_BYTES = FCAP * 4;
 
-// LDS: [main-loop split terms, two stages | (aliased) fix-up staging + pair
-// list] [norms, scales, scan]
+// LDS: [main-loop split terms, NSTAGE stages | (aliased) fix-up staging +
+// pair list] [norms, scales, scan]
 template <int NT>
 struct GramLds {
     static constexpr int STAGE = 2 * NT * TERM_BYTES;
-    static constexpr int MAIN = 2 * STAGE;
+    static constexpr int NSTAGE = 3;
+    static constexpr int MAIN = NSTAGE * STAGE;
     static constexpr int FIX = FIX_BYTES + LIST_BYTES;
     static constexpr int REGION = MAIN > FIX ? MAIN : FIX;
     static constexpr int BYTES = REGION + 4 * GBM * 4 + 16 * 4;
@@ -190,10 +189,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     typedef typename Split<SPLIT>::vec vec;
     constexpr int NT = Split<SPLIT>::NT;
     typedef GramLds<NT> L;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
-    // split term t of side s (0 = A, rows j; 1 = B, rows i) in stage g
-    unsigned short *lterms = reinterpret_cast<unsigned short *>(smem);
-    auto tS = [&](int g, int s, int t) { return lterms + ((g * 2 + s) * NT + t) * GBM * GLDK; };
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[L::BYTES];
     float *fS = reinterpret_cast<float *>(smem);          // fix-up staging (aliases the terms)
     unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
     float *nA = reinterpret_cast<float *>(smem + L::REGION);  // row norms, A side (j)
@@ -228,44 +224,45 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < n ? args.iscale[r] : 1.0f;
     }
 
-    // ---- main loop: 16 B term loads -> LDS (two stages), MFMA products -----
-    // Each thread stages one 16 B unit per side and term: row tid >> 1,
-    // features (tid & 1) * 8..+7 of the chunk.  Rows past n are clamped to
-    // row n-1: they only feed pairs that are never stored.
-    // (plain macros with constant indices keep the staging registers out of
-    // private memory)
+    // ---- main loop: direct global -> LDS loads, NSTAGE-deep, MFMA products -
+    // LDS chunk image: for stage g, side s (0 = A, rows j; 1 = B, rows i) and
+    // term t, 256 rows x 32 B; the two 16 B halves of row R are swapped when
+    // swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
+    // 8 or 16 consecutive rows hit distinct banks without padding.
+    // global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so
+    // each wave instruction fills 32 rows: wave w fills rows 32w..32w+31 of
+    // every (side, term) block.  Rows past n are clamped to row n-1: they only
+    // feed pairs that are never stored.
+    typedef __attribute__((address_space(3))) void lds_void_t;
+    typedef const __attribute__((address_space(1))) void gbl_void_t;
     const int64_t term_stride = n * kp;
     const unsigned short *gsA, *gsB;
     {
-        int64_t r = j0 + (tid >> 1);
-        gsA = args.terms + (r < n ? r : n - 1) * kp + (tid & 1) * 8;
-        r = i0 + (tid >> 1);
-        gsB = args.terms + (r < n ? r : n - 1) * kp + (tid & 1) * 8;
+        const int row = wave * 32 + (lane >> 1);
+        const int half = (lane & 1) ^ (((row >> 2) ^ (row >> 3)) & 1);
+        int64_t r = j0 + row;
+        gsA = args.terms + (r < n ? r : n - 1) * kp + half * 8;
+        r = i0 + row;
+        gsB = args.terms + (r < n ? r : n - 1) * kp + half * 8;
     }
-    const int soff = (tid >> 1) * GLDK + (tid & 1) * 8;
-    uint4 pA0, pA1, pA2, pB0, pB1, pB2;
-#define GT_GLOAD(kb)                                                                       \
-    do {                                                                                   \
-        pA0 = *reinterpret_cast<const uint4 *>(gsA + (kb));                                \
-        pB0 = *reinterpret_cast<const uint4 *>(gsB + (kb));                                \
-        pA1 = *reinterpret_cast<const uint4 *>(gsA + term_stride + (kb));                  \
-        pB1 = *reinterpret_cast<const uint4 *>(gsB + term_stride + (kb));                  \
-        if (NT == 3) {                                                                     \
-            pA2 = *reinterpret_cast<const uint4 *>(gsA + 2 * term_stride + (kb));          \
-            pB2 = *reinterpret_cast<const uint4 *>(gsB + 2 * term_stride + (kb));          \
-        }                                                                                  \
-    } while (0)
-#define GT_SSTORE(g)                                                                       \
-    do {                                                                                   \
-        *reinterpret_cast<uint4 *>(tS(g, 0, 0) + soff) = pA0;                              \
-        *reinterpret_cast<uint4 *>(tS(g, 1, 0) + soff) = pB0;                              \
-        *reinterpret_cast<uint4 *>(tS(g, 0, 1) + soff) = pA1;                              \
-        *reinterpret_cast<uint4 *>(tS(g, 1, 1) + soff) = pB1;                              \
-        if (NT == 3) {                                                                     \
-            *reinterpret_cast<uint4 *>(tS(g, 0, NT - 1) + soff) = pA2;                     \
-            *reinterpret_cast<uint4 *>(tS(g, 1, NT - 1) + soff) = pB2;                     \
-        }                                                                                  \
-    } while (0)
+    unsigned char *const lbase = smem + wave * 1024;   // + block/stage offsets
+    auto issue = [&](int chunk) {
+        const int g = chunk % L::NSTAGE;
+        const int64_t kb = (int64_t)chunk * GBK;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            __builtin_amdgcn_global_load_lds(
+                (gbl_void_t *)(gsA + t * term_stride + kb),
+                (lds_void_t *)(lbase + g * L::STAGE + (0 * NT + t) * TERM_BYTES), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (gbl_void_t *)(gsB + t * term_stride + kb),
+                (lds_void_t *)(lbase + g * L::STAGE + (1 * NT + t) * TERM_BYTES), 16, 0, 0);
+        }
+    };
+    // operand read offset of this lane (rows differ by multiples of 32 between
+    // blocks, which leaves swz unchanged)
+    const int lrow = lane & 31;
+    const int roff = lrow * GROWB + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
 
     f32x16_t acc[2][4];
 #pragma unroll
@@ -276,27 +273,42 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
     const int nchunks = (int)(kp / GBK);
-    const int koff = (lane >> 5) * 8;
-    GT_GLOAD(0);
-    GT_SSTORE(0);
-    if (nchunks > 1) GT_GLOAD(GBK);
-    __syncthreads();
+    // prologue: chunks 0 .. NSTAGE-2 in flight
+#pragma unroll
+    for (int c = 0; c < L::NSTAGE - 1; ++c)
+        if (c < nchunks) issue(c);
     for (int c = 0; c < nchunks; ++c) {
-        const int g = c & 1;
+        // chunk c landed when at most the later chunks' 2*NT loads per wave
+        // are outstanding (vector memory loads complete in order)
+        if (c + 1 < nchunks) {
+            if (NT == 3)
+                asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+        } else {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        // every wave's part of chunk c is in LDS, and every wave finished
+        // chunk c-1, whose stage the next issue overwrites.  A bare s_barrier:
+        // __syncthreads() would also drain the later chunks' loads.
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + L::NSTAGE - 1 < nchunks) issue(c + L::NSTAGE - 1);
+        const unsigned char *sg = smem + (c % L::NSTAGE) * L::STAGE + roff;
         vec av[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int a = 0; a < 2; ++a)
-                av[t][a] = *reinterpret_cast<const vec *>(
-                    tS(g, 0, t) + (wr * 64 + a * 32 + (lane & 31)) * GLDK + koff);
+                av[t][a] = *reinterpret_cast<const vec *>(sg + t * TERM_BYTES +
+                                                          (wr * 64 + a * 32) * GROWB);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             vec bv[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-                bv[t] = *reinterpret_cast<const vec *>(
-                    tS(g, 1, t) + (wc * 128 + b * 32 + (lane & 31)) * GLDK + koff);
+                bv[t] = *reinterpret_cast<const vec *>(sg + (NT + t) * TERM_BYTES +
+                                                       (wc * 128 + b * 32) * GROWB);
             // hh' first, then the cross terms in decreasing magnitude
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
@@ -310,14 +322,9 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
                 }
             }
         }
-        // The other stage was last read in chunk c-1, which every wave
-        // finished before the barrier that ended it.
-        if (c + 1 < nchunks) GT_SSTORE(g ^ 1);
-        __syncthreads();
-        if (c + 2 < nchunks) GT_GLOAD((int64_t)(c + 2) * GBK);
     }
-#undef GT_GLOAD
-#undef GT_SSTORE
+    // the fix-up below reuses the stage memory
+    __syncthreads();
 
     // ---- epilogue: Gram-form distance, or flag for exact recompute ---------
     // 32x32 accumulator layout: element r of lane l is row 8(r/4) + 4(l/32) +
@@ -371,26 +378,72 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     const int myoff = block_exclusive_scan(__popcll(fmask[0]) + __popcll(fmask[1]), scan_tmp, total);
     if (total == 0) return;
     const int64_t ldx = args.ldx;
-    // staging unit e = tid + 512 q: side e >> 11, row (e >> 3) & 255, features (e & 7) * 4
-    float4 fpre[FUNITS];
-    auto fload = [&](int64_t fb) {
+    // Staging unit e = tid + 512 q (q < FUNITS): side e >> 11, row
+    // (e >> 3) & 255, features (e & 7) * 4..+3 of a 32-feature chunk.  The
+    // accumulators are dead here, so their registers hold a ring of FRING
+    // chunks in flight (the pass is bound by load latency, not bandwidth).
+    constexpr int FRING = 4;
+    const float *fsrc[FUNITS];
+    int fcol[FUNITS];
 #pragma unroll
-        for (int q = 0; q < FUNITS; ++q) {
-            const int e = tid + GTHREADS * q;
-            int64_t g = ((e >> 11) ? i0 : j0) + ((e >> 3) & (GBM - 1));
-            g = g < n ? g : n - 1;
-            const int64_t kk = fb + (e & 7) * 4;
-            if (VEC4) {
-                fpre[q] = kk < k ? *reinterpret_cast<const float4 *>(args.x + g * ldx + kk)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                float v[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) v[c] = (kk + c < k) ? args.x[g * ldx + kk + c] : 0.0f;
-                fpre[q] = make_float4(v[0], v[1], v[2], v[3]);
-            }
-        }
-    };
+    for (int q = 0; q < FUNITS; ++q) {
+        const int e = tid + GTHREADS * q;
+        int64_t g = ((e >> 11) ? i0 : j0) + ((e >> 3) & (GBM - 1));
+        g = g < n ? g : n - 1;
+        fsrc[q] = args.x + g * ldx;
+        fcol[q] = (e & 7) * 4;
+    }
+    const int nf = (int)((k + FBK - 1) / FBK);
+    float4 fr0[FUNITS], fr1[FUNITS], fr2[FUNITS], fr3[FUNITS];
+#define GT_FLOAD(buf, chunk)                                                               \
+    do {                                                                                   \
+        const int64_t fb_ = (int64_t)(chunk) * FBK;                                        \
+        _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
+            const int64_t kk = fb_ + fcol[q];                                              \
+            if (VEC4) {                                                                    \
+                buf[q] = kk < k ? *reinterpret_cast<const float4 *>(fsrc[q] + kk)          \
+                                : make_float4(0.f, 0.f, 0.f, 0.f);                         \
+            } else {                                                                       \
+                float v_[4];                                                               \
+                _Pragma("unroll") for (int c_ = 0; c_ < 4; ++c_) v_[c_] =                  \
+                    (kk + c_ < k) ? fsrc[q][kk + c_] : 0.0f;                               \
+                buf[q] = make_float4(v_[0], v_[1], v_[2], v_[3]);                          \
+            }                                                                              \
+        }                                                                                  \
+    } while (0)
+    // one chunk: registers -> LDS, refill the registers FRING chunks ahead,
+    // then accumulate this thread's flagged pairs over the chunk
+#define GT_FSTEP(buf, chunk)                                                               \
+    do {                                                                                   \
+        __syncthreads();                                                                   \
+        _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
+            const int e = tid + GTHREADS * q;                                              \
+            *reinterpret_cast<float4 *>(fS + ((e >> 11) * GBM + ((e >> 3) & (GBM - 1))) *  \
+                                                 FLD + (e & 7) * 4) = buf[q];              \
+        }                                                                                  \
+        __syncthreads();                                                                   \
+        if ((chunk) + FRING < nf) GT_FLOAD(buf, (chunk) + FRING);                          \
+        _Pragma("unroll") for (int c = 0; c < FPER; ++c) {                                 \
+            if (tid + GTHREADS * c < cnt) {                                                \
+                const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD); \
+                const float4 *rb =                                                         \
+                    reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD); \
+                float part = 0.0f;                                                         \
+                _Pragma("unroll") for (int q = 0; q < FBK / 4; ++q) {                      \
+                    const float4 u = ra[q], v = rb[q];                                     \
+                    float t = u.x - v.x;                                                   \
+                    part = fmaf(t, t, part);                                               \
+                    t = u.y - v.y;                                                         \
+                    part = fmaf(t, t, part);                                               \
+                    t = u.z - v.z;                                                         \
+                    part = fmaf(t, t, part);                                               \
+                    t = u.w - v.w;                                                         \
+                    part = fmaf(t, t, part);                                               \
+                }                                                                          \
+                sum[c] += part;                                                            \
+            }                                                                              \
+        }                                                                                  \
+    } while (0)
     for (int base = 0; base < total; base += FCAP) {
         {
             int rank = myoff;
@@ -415,45 +468,21 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         unsigned ent[FPER];
 #pragma unroll
         for (int c = 0; c < FPER; ++c) sum[c] = 0.0f;
-        fload(0);
+        GT_FLOAD(fr0, 0);
+        if (nf > 1) GT_FLOAD(fr1, 1);
+        if (nf > 2) GT_FLOAD(fr2, 2);
+        if (nf > 3) GT_FLOAD(fr3, 3);
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
             const int e = tid + GTHREADS * c;
             ent[c] = e < cnt ? list[e] : 0u;
         }
-        for (int64_t fb = 0; fb < k; fb += FBK) {
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < FUNITS; ++q) {
-                const int e = tid + GTHREADS * q;
-                *reinterpret_cast<float4 *>(fS + ((e >> 11) * GBM + ((e >> 3) & (GBM - 1))) * FLD +
-                                            (e & 7) * 4) = fpre[q];
-            }
-            __syncthreads();
-            if (fb + FBK < k) fload(fb + FBK);
-#pragma unroll
-            for (int c = 0; c < FPER; ++c) {
-                if (tid + GTHREADS * c < cnt) {
-                    const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD);
-                    const float4 *rb =
-                        reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD);
-                    float part = 0.0f;
-#pragma unroll
-                    for (int q = 0; q < FBK / 4; ++q) {
-                        const float4 u = ra[q], v = rb[q];
-                        float t = u.x - v.x;
-                        part = fmaf(t, t, part);
-                        t = u.y - v.y;
-                        part = fmaf(t, t, part);
-                        t = u.z - v.z;
-                        part = fmaf(t, t, part);
-                        t = u.w - v.w;
-                        part = fmaf(t, t, part);
-                    }
-                    sum[c] += part;
-                }
-            }
+        for (int ch = 0; ch < nf; ch += FRING) {
+            GT_FSTEP(fr0, ch);
+            if (ch + 1 < nf) GT_FSTEP(fr1, ch + 1);
+            if (ch + 2 < nf) GT_FSTEP(fr2, ch + 2);
+            if (ch + 3 < nf) GT_FSTEP(fr3, ch + 3);
         }
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
@@ -470,6 +499,8 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         }
         __syncthreads();
     }
+#undef GT_FSTEP
+#undef GT_FLOAD
 }
 
 // ||x_i||^2 per row (and, for FP16, the power-of-two row scale and its
