@@ -1,0 +1,310 @@
+// gt_backward.hip -- gradients of the all-pairs distance vector.
+//
+// gputools has no backward; these kernels differentiate the forward formulas
+// of inst/cuda/distance.cu (euclidean :40-78, minkowski :453-493).  For the
+// packed distance vector D and an upstream gradient G = dL/dD (packed, R
+// 'dist' order):
+//   dL/dx_ic = sum_{j != i} G_ij * dd_ij/dx_ic,
+//   euclidean   dd_ij/dx_ic = (x_ic - x_jc) / d_ij
+//   minkowski   dd_ij/dx_ic = sign(t) |t|^(p-1) / d_ij^(p-1),  t = x_ic - x_jc
+// and a pair with d_ij == 0 contributes 0 (subgradient).
+//
+// Row-owner sweep: a workgroup owns 64 rows i and one block of up to 128
+// features, and visits every 64-row block j.  Per (i, j) tile it recomputes
+// d_ij exactly (FP32 differences, two-level summation over 32-feature LDS
+// chunks; D is never stored), forms w_ij = G_ij / d_ij^(p-1) in LDS, then
+// accumulates dX_i over its features.  Every dX row is written by exactly one
+// workgroup: no atomics, deterministic.
+//
+// The same kernel computes the metric-MDS stress S = sum_{i<j} (d_ij - delta_ij)^2
+// of an embedding Y and its gradient (G_ij = 2 (d_ij - delta_ij)); S is summed
+// in FP64 from per-workgroup partials (the only FP64 on the device).
+#include "gt_common.h"
+#include "gt_internal.h"
+
+namespace gt {
+namespace {
+
+constexpr int BI = 64;          // owned rows per workgroup
+constexpr int BJ = 64;          // rows per visited block
+constexpr int BTH = 256;        // threads
+constexpr int DKC = 32;         // features per distance chunk
+constexpr int DLD = BI + 4;     // LDS row length of the k-major distance chunks
+constexpr int KBMAX = 128;      // largest feature block per workgroup
+constexpr int WLD = BJ + 1;     // LDS row length of the weight tile
+
+enum { MODE_GRAD = 0, MODE_MDS = 1 };
+enum { BW_EUCLID = 0, BW_MINK3 = 1, BW_MINKP = 2 };
+
+struct BwArgs {
+    const float *x;
+    int64_t n, k, ldx;
+    const float *g;        // MODE_GRAD: packed dL/dD; MODE_MDS: packed delta
+    float *dx;
+    int64_t lddx;
+    double *partials;      // MODE_MDS: one per (row block) workgroup
+    float p, pm1;          // p and p - 1
+};
+
+template <int OP>
+__device__ __forceinline__ float dterm(float t)
+{
+    // contribution of one feature difference to the distance accumulator
+    if (OP == BW_EUCLID) return t * t;
+    const float a = fabsf(t);
+    if (OP == BW_MINK3) return a * a * a;
+    return 0.0f;  // BW_MINKP handled by the caller (needs p)
+}
+
+// KB: features per workgroup (4, 32 or 128, chosen from k on the host).
+template <int OP, int MODE, int KB>
+__global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
+{
+    constexpr int XLD = KB + 1;   // LDS row length of the x_J feature block
+    __shared__ float xi_s[DKC][DLD];
+    __shared__ float xj_s[DKC][DLD];
+    __shared__ float w_s[BI][WLD];
+    __shared__ float xb_s[BJ][XLD];
+    __shared__ double red_s[BTH / 64];
+
+    const int tid = threadIdx.x;
+    const int64_t n = args.n, k = args.k, ldx = args.ldx;
+    const int64_t i0 = (int64_t)blockIdx.x * BI;
+    const int64_t f0 = (int64_t)blockIdx.y * KB;
+    const int64_t fcount = (k - f0) < KB ? (k - f0) : KB;
+
+    // distance phase mapping: 4 x 4 pairs per thread
+    const int ty = tid >> 4, tx = tid & 15;
+    // accumulation phase mapping: row tid >> 2, features (tid & 3) + 4 m
+    const int arow = tid >> 2, asub = tid & 3;
+    const int64_t gi = i0 + arow;
+    constexpr int FPT = KB / 4;   // features per thread (KB >= 4)
+    float xi_r[FPT], acc[FPT];
+    float rowsum = 0.0f;
+#pragma unroll
+    for (int m = 0; m < FPT; ++m) {
+        const int64_t c = f0 + asub + 4 * m;
+        xi_r[m] = (gi < n && c < k && asub + 4 * m < fcount) ? args.x[gi * ldx + c] : 0.0f;
+        acc[m] = 0.0f;
+    }
+    double stress = 0.0;
+
+    for (int64_t j0 = 0; j0 < n; j0 += BJ) {
+        // ---- d_ij for the 64 x 64 tile, exact FP32 differences ------------
+        float dsum[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) dsum[a][b] = 0.0f;
+        for (int64_t kb = 0; kb < k; kb += DKC) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < (BI * DKC) / BTH; ++q) {
+                const int e = tid + BTH * q;
+                const int row = e >> 5, kc = e & 31;
+                const int64_t kk = kb + kc;
+                const int64_t ri = i0 + row, rj = j0 + row;
+                xi_s[kc][row] = (ri < n && kk < k) ? args.x[ri * ldx + kk] : 0.0f;
+                xj_s[kc][row] = (rj < n && kk < k) ? args.x[rj * ldx + kk] : 0.0f;
+            }
+            __syncthreads();
+            float part[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) part[a][b] = 0.0f;
+#pragma unroll 4
+            for (int kc = 0; kc < DKC; ++kc) {
+                const float4 vi = *reinterpret_cast<const float4 *>(&xi_s[kc][ty * 4]);
+                const float4 vj = *reinterpret_cast<const float4 *>(&xj_s[kc][tx * 4]);
+                const float ai[4] = {vi.x, vi.y, vi.z, vi.w};
+                const float aj[4] = {vj.x, vj.y, vj.z, vj.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const float t = ai[a] - aj[b];
+                        if (OP == BW_MINKP)
+                            part[a][b] += exp2f(args.p * log2f(fabsf(t)));
+                        else
+                            part[a][b] += dterm<OP>(t);
+                    }
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) dsum[a][b] += part[a][b];
+        }
+        // ---- weights w_ij = G_ij / d_ij^(p-1) -------------------------------
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int li = ty * 4 + a, lj = tx * 4 + b;
+                const int64_t i = i0 + li, j = j0 + lj;
+                float w = 0.0f;
+                if (i < n && j < n && i != j) {
+                    const float s = dsum[a][b];
+                    const float d = (OP == BW_EUCLID) ? sqrtf(s)
+                                    : (OP == BW_MINK3) ? cbrtf(s)
+                                                       : powf(s, 1.0f / args.p);
+                    const int64_t hi = i > j ? i : j, lo = i > j ? j : i;
+                    const float gv = args.g[packed_base(n, lo) + hi];
+                    if (d > 0.0f) {
+                        float gij;
+                        if (MODE == MODE_MDS) {
+                            const float r = d - gv;     // gv = delta_ij
+                            stress += (double)r * (double)r;
+                            gij = 2.0f * r;
+                        } else {
+                            gij = gv;
+                        }
+                        // d^(p-1): p = 2 -> d, p = 3 -> d^2
+                        const float dp = (OP == BW_EUCLID) ? d
+                                         : (OP == BW_MINK3) ? d * d
+                                                            : powf(d, args.pm1);
+                        w = gij / dp;
+                    } else if (MODE == MODE_MDS) {
+                        const float r = -gv;            // d == 0
+                        stress += (double)r * (double)r;
+                    }
+                }
+                w_s[li][lj] = w;
+            }
+        // ---- x_J feature block ----------------------------------------------
+        for (int e = tid; e < BJ * KB; e += BTH) {
+            const int row = e / KB, c = e % KB;
+            const int64_t rj = j0 + row;
+            xb_s[row][c] = (rj < n && c < fcount) ? args.x[rj * ldx + f0 + c] : 0.0f;
+        }
+        __syncthreads();
+        // ---- dX_i += sum_j w_ij phi(x_i - x_j) ------------------------------
+        const int jmax = (n - j0) < BJ ? (int)(n - j0) : BJ;
+        for (int lj = 0; lj < jmax; ++lj) {
+            const float w = w_s[arow][lj];
+            if (OP == BW_EUCLID) {
+                rowsum += w;
+#pragma unroll
+                for (int m = 0; m < FPT; ++m) acc[m] = fmaf(-w, xb_s[lj][asub + 4 * m], acc[m]);
+            } else {
+#pragma unroll
+                for (int m = 0; m < FPT; ++m) {
+                    const float t = xi_r[m] - xb_s[lj][asub + 4 * m];
+                    const float a = fabsf(t);
+                    // sign(t) |t|^(p-1); 0 at t == 0 (p >= 1)
+                    const float ph = (OP == BW_MINK3) ? a * t
+                                     : (a > 0.0f ? copysignf(exp2f(args.pm1 * log2f(a)), t) : 0.0f);
+                    acc[m] = fmaf(w, ph, acc[m]);
+                }
+            }
+        }
+    }
+
+    if (gi < n && args.dx) {
+#pragma unroll
+        for (int m = 0; m < FPT; ++m) {
+            if (asub + 4 * m < fcount) {
+                const float v = (OP == BW_EUCLID) ? fmaf(xi_r[m], rowsum, acc[m]) : acc[m];
+                args.dx[gi * args.lddx + f0 + asub + 4 * m] = v;
+            }
+        }
+    }
+
+    if (MODE == MODE_MDS && blockIdx.y == 0) {
+        // each unordered pair was seen by both of its row owners: S = sum / 2
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) stress += __shfl_xor(stress, d, 64);
+        __syncthreads();
+        if ((tid & 63) == 0) red_s[tid >> 6] = stress;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < BTH / 64; ++w) s += red_s[w];
+            args.partials[blockIdx.x] = 0.5 * s;
+        }
+    }
+}
+
+// Sum of the per-workgroup FP64 partials, fixed order (deterministic).
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *partials, int64_t count,
+                                                              double *out)
+{
+    __shared__ double s_[256];
+    double s = 0.0;
+    for (int64_t e = threadIdx.x; e < count; e += 256) s += partials[e];
+    s_[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 128; d >= 1; d >>= 1) {
+        if ((int)threadIdx.x < d) s_[threadIdx.x] += s_[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s_[0];
+}
+
+template <int MODE, int KB>
+hipError_t launch_kb(int op, BwArgs &args, hipStream_t stream)
+{
+    dim3 grid((unsigned)((args.n + BI - 1) / BI), (unsigned)((args.k + KB - 1) / KB));
+    if (op == BW_EUCLID)
+        hipLaunchKernelGGL((backward_kernel<BW_EUCLID, MODE, KB>), grid, dim3(BTH), 0, stream, args);
+    else if (op == BW_MINK3)
+        hipLaunchKernelGGL((backward_kernel<BW_MINK3, MODE, KB>), grid, dim3(BTH), 0, stream, args);
+    else
+        hipLaunchKernelGGL((backward_kernel<BW_MINKP, MODE, KB>), grid, dim3(BTH), 0, stream, args);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_mode(int op, BwArgs &args, hipStream_t stream)
+{
+    if (args.k <= 4) return launch_kb<MODE, 4>(op, args, stream);
+    if (args.k <= 32) return launch_kb<MODE, 32>(op, args, stream);
+    return launch_kb<MODE, KBMAX>(op, args, stream);
+}
+
+}  // namespace
+
+int64_t mds_partials_count(int64_t n) { return (n + BI - 1) / BI; }
+
+hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t ldx, float p,
+                           const float *g_packed, float *dx, int64_t lddx, hipStream_t stream)
+{
+    if (n <= 0 || k <= 0) return hipSuccess;
+    BwArgs args;
+    args.x = x;
+    args.n = n;
+    args.k = k;
+    args.ldx = ldx;
+    args.g = g_packed;
+    args.dx = dx;
+    args.lddx = lddx;
+    args.partials = nullptr;
+    args.p = p;
+    args.pm1 = p - 1.0f;
+    const int bop = (op == OP_EUCLID) ? BW_EUCLID : (op == OP_MINK3) ? BW_MINK3 : BW_MINKP;
+    return launch_mode<MODE_GRAD>(bop, args, stream);
+}
+
+hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy, const float *delta,
+                      float *dy, int64_t lddy, double *stress, double *partials,
+                      hipStream_t stream)
+{
+    BwArgs args;
+    args.x = y;
+    args.n = n;
+    args.k = q;
+    args.ldx = ldy;
+    args.g = delta;
+    args.dx = dy;
+    args.lddx = lddy;
+    args.partials = partials;
+    args.p = 2.0f;
+    args.pm1 = 1.0f;
+    hipError_t e = launch_mode<MODE_MDS>(BW_EUCLID, args, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, stream, partials,
+                       mds_partials_count(n), stress);
+    return hipGetLastError();
+}
+
+}  // namespace gt

@@ -486,18 +486,59 @@ void rgetDevice(int *device)
 
 }  // extern "C"
 
-// ------------------------- not yet implemented ----------------------------
+// ------------------------------ gradients ---------------------------------
 extern "C" {
-int gt_dist_backward(const float *, int64_t, int64_t, int64_t, int, float, const float *, float *,
-                     int64_t, void *)
+int gt_dist_backward(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+                     const float *g_packed, float *dx, int64_t lddx, void *stream)
 {
-    return fail(GT_ERR_UNSUPPORTED, "gt_dist_backward: not built yet");
+    if (method != GT_EUCLIDEAN && method != GT_MINKOWSKI)
+        return fail(GT_ERR_UNSUPPORTED, "backward exists for euclidean and minkowski (method %d)",
+                    method);
+    if (method == GT_MINKOWSKI && !(p >= 1.0f && isfinite(p)))
+        return fail(GT_ERR_INVALID, "minkowski backward needs a finite p >= 1 (p = %g)", (double)p);
+    if (n < 0 || k < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (ldx < k || lddx < k) return fail(GT_ERR_INVALID, "leading dimension < k");
+    if (n == 0 || k == 0) return GT_OK;
+    if (x == nullptr || dx == nullptr || (n > 1 && g_packed == nullptr))
+        return fail(GT_ERR_INVALID, "null pointer");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    if (n == 1) {
+        GT_HIP(gt::launch_fill(dx, k, 0.0f, hs));
+        return GT_OK;
+    }
+    int op = gt::OP_MINKP;
+    if (method == GT_EUCLIDEAN || p == 2.0f) op = gt::OP_EUCLID;
+    else if (p == 3.0f) op = gt::OP_MINK3;
+    GT_HIP(gt::launch_backward(op, x, n, k, ldx, p, g_packed, dx, lddx, hs));
+    return GT_OK;
 }
-int gt_mds_stress(const float *, int64_t, int64_t, int64_t, const float *, float *, int64_t,
-                  double *, void *)
+
+int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy, const float *delta_packed,
+                  float *dy, int64_t lddy, double *stress_dev, void *stream)
 {
-    return fail(GT_ERR_UNSUPPORTED, "gt_mds_stress: not built yet");
+    if (n < 0 || q < 1) return fail(GT_ERR_INVALID, "need n >= 0 and q >= 1");
+    if (ldy < q || (dy != nullptr && lddy < q)) return fail(GT_ERR_INVALID, "leading dimension < q");
+    if (stress_dev == nullptr) return fail(GT_ERR_INVALID, "null stress pointer");
+    if (n > 1 && (y == nullptr || delta_packed == nullptr)) return fail(GT_ERR_INVALID, "null pointer");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const int64_t parts = n > 1 ? gt::mds_partials_count(n) : 1;
+    Scratch ws;
+    int s = ws.acquire(round_up((size_t)parts * sizeof(double), 256), hs);
+    if (s != GT_OK) return s;
+    hipError_t e;
+    if (n < 2) {
+        // a fill kernel (not a memset node) so that the call stays capturable
+        e = gt::launch_fill(reinterpret_cast<float *>(stress_dev), 2, 0.0f, hs);
+        if (e == hipSuccess && dy != nullptr && n == 1) e = gt::launch_fill(dy, q, 0.0f, hs);
+    } else {
+        e = gt::launch_mds(y, n, q, ldy, delta_packed, dy, lddy, stress_dev,
+                           static_cast<double *>(ws.ptr), hs);
+    }
+    const int r = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "gt_mds_stress");
+    return r;
 }
+
 int gt_dist_batched(const float *, const int64_t *, const int64_t *, const int64_t *, int64_t,
                     float *, const int64_t *, int, int, void *, size_t, void *)
 {

@@ -50,12 +50,14 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
                        int64_t ldx, float *out, int64_t ldd, void *workspace,
                        hipStream_t stream);
 
-// Backward (dL/dX) and fused MDS stress.
+// Backward (dL/dX) and fused MDS stress.  op: OP_EUCLID, OP_MINK3 or OP_MINKP.
 hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t ldx,
                            float p, const float *g_packed, float *dx, int64_t lddx,
                            hipStream_t stream);
+// partials: mds_partials_count(n) doubles of workspace.
+int64_t mds_partials_count(int64_t n);
 hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy,
                       const float *delta, float *dy, int64_t lddy, double *stress,
-                      hipStream_t stream);
+                      double *partials, hipStream_t stream);
 
 }  // namespace gt
