@@ -68,9 +68,10 @@ SIGNATURES = {
     "gt_free": (c_int, [c_p]),
     "gt_dist_backward": (c_int, [c_p, c_i64, c_i64, c_i64, c_int, c_float, c_p, c_p, c_i64, c_p]),
     "gt_mds_stress": (c_int, [c_p, c_i64, c_i64, c_i64, c_p, c_p, c_i64, c_p, c_p]),
-    "gt_dist_batched": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_int, c_int, c_p, c_size,
+    "gt_dist_batched": (c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_int, c_int, c_p, c_i64,
                                 c_p]),
-    "gt_dist_batched_workspace_size": (c_int, [c_p, c_p, c_i64, c_int, ctypes.POINTER(c_size)]),
+    "gt_dist_batched_schedule": (c_int, [c_p, c_p, c_i64, c_int, c_p, c_i64,
+                                         ctypes.POINTER(c_i64)]),
     "gt_batched_tile_count": (c_int, []),
 }
 

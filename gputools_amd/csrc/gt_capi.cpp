@@ -17,7 +17,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <limits>
+#include <vector>
 
 namespace {
 
@@ -539,14 +541,66 @@ int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy, const float
     return r;
 }
 
-int gt_dist_batched(const float *, const int64_t *, const int64_t *, const int64_t *, int64_t,
-                    float *, const int64_t *, int, int, void *, size_t, void *)
+int gt_batched_tile_count(void) { return 3; }
+
+int gt_dist_batched_schedule(const int64_t *sizes_host, const int64_t *dims_host, int64_t nsets,
+                             int tile, int64_t *schedule_host, int64_t capacity, int64_t *count)
 {
-    return fail(GT_ERR_UNSUPPORTED, "gt_dist_batched: not built yet");
+    if (count == nullptr) return fail(GT_ERR_INVALID, "null count pointer");
+    *count = 0;
+    const int bm = gt::batched_tile_side(tile);
+    if (bm == 0) return fail(GT_ERR_INVALID, "unknown tile shape %d", tile);
+    if (nsets < 0 || nsets > (int64_t)INT32_MAX) return fail(GT_ERR_INVALID, "bad number of sets");
+    if (nsets > 0 && (sizes_host == nullptr || dims_host == nullptr))
+        return fail(GT_ERR_INVALID, "null descriptor pointer");
+    // entries: (set, I, J) for every lower tile J <= I of every set with >= 2
+    // rows and >= 1 feature; cost order: more features first, then bigger sets
+    struct Ent { int64_t key, d, n; };
+    std::vector<Ent> ents;
+    for (int64_t s = 0; s < nsets; ++s) {
+        const int64_t n = sizes_host[s], d = dims_host[s];
+        if (n < 0 || d < 0) return fail(GT_ERR_INVALID, "set %lld has a negative size", (long long)s);
+        if (n < 2) continue;
+        if (d < 1) return fail(GT_ERR_INVALID, "set %lld has no features", (long long)s);
+        const int64_t T = (n + bm - 1) / bm;
+        if (T > 65535) return fail(GT_ERR_INVALID, "set %lld is too large for batching", (long long)s);
+        for (int64_t I = 0; I < T; ++I)
+            for (int64_t J = 0; J <= I; ++J) ents.push_back({(s << 32) | (I << 16) | J, d, n});
+    }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        return a.d != b.d ? a.d > b.d : a.n > b.n;
+    });
+    *count = (int64_t)ents.size();
+    if (schedule_host != nullptr && capacity >= *count)
+        for (size_t e = 0; e < ents.size(); ++e) schedule_host[e] = ents[e].key;
+    return GT_OK;
 }
-int gt_dist_batched_workspace_size(const int64_t *, const int64_t *, int64_t, int, size_t *)
+
+int gt_dist_batched(const float *x, const int64_t *x_offsets, const int64_t *sizes,
+                    const int64_t *dims, int64_t nsets, float *out, const int64_t *out_offsets,
+                    int method, int tile, const int64_t *schedule, int64_t count, void *stream)
 {
-    return fail(GT_ERR_UNSUPPORTED, "gt_dist_batched: not built yet");
+    int op;
+    switch (method) {
+    case GT_EUCLIDEAN: op = gt::OP_EUCLID; break;
+    case GT_MANHATTAN: op = gt::OP_MANHATTAN; break;
+    case GT_MAXIMUM: op = gt::OP_MAXIMUM; break;
+    default:
+        return fail(GT_ERR_UNSUPPORTED, "batched path exists for euclidean, manhattan, maximum");
+    }
+    if (gt::batched_tile_side(tile) == 0) return fail(GT_ERR_INVALID, "unknown tile shape %d", tile);
+    if (count < 0 || nsets < 0) return fail(GT_ERR_INVALID, "negative count");
+    if (count == 0) return GT_OK;
+    if (!x || !x_offsets || !sizes || !dims || !out || !out_offsets || !schedule)
+        return fail(GT_ERR_INVALID, "null pointer");
+    int dev = 0, cus = 0;
+    GT_HIP(hipGetDevice(&dev));
+    GT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t grid = (int64_t)cus * gt::batched_resident_blocks(op, tile);
+    if (grid > count) grid = count;
+    if (grid < 1) grid = 1;
+    GT_HIP(gt::launch_batched(op, tile, x, x_offsets, sizes, dims, out_offsets, out, schedule, count,
+                              nsets, (int)grid, static_cast<hipStream_t>(stream)));
+    return GT_OK;
 }
-int gt_batched_tile_count(void) { return 0; }
 }

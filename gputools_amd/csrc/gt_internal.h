@@ -60,4 +60,12 @@ hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy,
                       const float *delta, float *dy, int64_t lddy, double *stress,
                       double *partials, hipStream_t stream);
 
+// Batched persistent kernel.  tile: 1..3 (32, 64, 128 pairs per side).
+int batched_tile_side(int tile);
+int batched_resident_blocks(int op, int tile);
+hipError_t launch_batched(int op, int tile, const float *x, const int64_t *x_off,
+                          const int64_t *sizes, const int64_t *dims, const int64_t *out_off,
+                          float *out, const int64_t *schedule, int64_t count, int64_t nsets,
+                          int grid, hipStream_t stream);
+
 }  // namespace gt

@@ -176,23 +176,26 @@ int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy,
                   double *stress_dev, void *stream);
 
 /* Batched distances over many independent point sets in one persistent
- * launch.  Set s has sizes[s] rows of dims[s] features starting at element
- * x_offsets[s] of x (row stride dims[s]); its packed output starts at element
- * out_offsets[s] of out.  All descriptor arrays are device arrays of nsets
- * int64.  Supported methods: euclidean, manhattan, maximum (FP32 path).
- * 'tile' selects the tile shape (0 = default; see gt_batched_tile_count). */
-int gt_dist_batched(const float *x, const int64_t *x_offsets, const int64_t *sizes,
-                    const int64_t *dims, int64_t nsets, float *out,
-                    const int64_t *out_offsets, int method, int tile,
-                    void *workspace, size_t workspace_bytes, void *stream);
-
-/* Workspace bytes gt_dist_batched needs (tile schedule), given the host copy
- * of sizes/dims. */
-int gt_dist_batched_workspace_size(const int64_t *sizes_host, const int64_t *dims_host,
-                                   int64_t nsets, int tile, size_t *bytes);
-
-/* Number of tile shapes gt_dist_batched offers (ids 1..count). */
+ * launch (no counterpart in the reference, which computes one matrix per
+ * distance() call, src/distance.cpp:259-316).  Set s has sizes[s] rows of
+ * dims[s] features starting at element x_offsets[s] of x (row stride
+ * dims[s]); its packed output ('dist' order) starts at element out_offsets[s]
+ * of out.  Methods: euclidean, manhattan, maximum (FP32 path).
+ *
+ * gt_dist_batched_schedule builds, on the host, the tile schedule for the
+ * given host copies of sizes and dims and a tile shape (1..gt_batched_tile_count():
+ * 32, 64 or 128 pairs per tile side): *count receives the number of int64
+ * entries; they are written to schedule_host when capacity allows.
+ * gt_dist_batched takes the descriptor arrays and that schedule in device
+ * memory, allocates nothing and synchronises nothing, so it can be captured
+ * in a HIP graph and replayed.  Entries that do not describe a tile of a set
+ * are skipped. */
 int gt_batched_tile_count(void);
+int gt_dist_batched_schedule(const int64_t *sizes_host, const int64_t *dims_host, int64_t nsets,
+                             int tile, int64_t *schedule_host, int64_t capacity, int64_t *count);
+int gt_dist_batched(const float *x, const int64_t *x_offsets, const int64_t *sizes,
+                    const int64_t *dims, int64_t nsets, float *out, const int64_t *out_offsets,
+                    int method, int tile, const int64_t *schedule, int64_t count, void *stream);
 
 #ifdef __cplusplus
 }
