@@ -48,6 +48,13 @@ def parse():
                     help="rows whose all pairs form the CPU-baseline / error sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-paths", action="store_true",
+                    help="skip the secondary paths (other methods, batched, backward, MDS)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 2 GiB device-to-device copy first (reference kernel of known "
+                         "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--precisions", default=None,
+                    help="comma list of paths to time (default: the headline and fp32)")
     return ap.parse_args()
 
 
@@ -89,6 +96,104 @@ def sampled_pairs(n, rows):
     return r, idx
 
 
+def _timed(fn, steps, warmup, torch, stream=None):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = stream or torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(steps):
+        fn()
+    b.record(st)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def _valu_roof(flops, ms):
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(tf, 2), "peak": 157.3, "unit": "TFLOP/s",
+            "frac": round(tf / 157.3, 4)}
+
+
+def secondary_paths(torch, gt, wl, steps=3, warmup=1):
+    """Timed secondary paths, each with its algorithmic work and roofline."""
+    out = {}
+    # all six methods, 8192 x 256 (FP32 VALU kernels; binary on bit rows)
+    n, k = 8192, 256
+    pairs = n * (n - 1) // 2
+    meth = {}
+    flops_per_feature = {"euclidean": 3, "maximum": 2, "manhattan": 2, "canberra": 4,
+                         "minkowski": 3}
+    for m in ("euclidean", "maximum", "manhattan", "canberra", "binary", "minkowski"):
+        x = torch.from_numpy(wl.method_input(m, n, k, seed=2)).cuda()
+        o = torch.empty(pairs, dtype=torch.float32, device="cuda")
+        p = 3.0 if m == "minkowski" else 2.0
+        ms = _timed(lambda: gt.dist(x, m, p, out=o), steps, warmup, torch)
+        rec = {"ms": round(ms, 4), "pairs_per_s": round(pairs / (ms * 1e-3), 1)}
+        if m == "binary":
+            gbs = (4.0 * pairs + 4.0 * n * k) / (ms * 1e-3) / 1e9
+            rec["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0,
+                               "unit": "GB/s", "frac": round(gbs / 8000.0, 4)}
+        else:
+            rec["roofline"] = _valu_roof(flops_per_feature[m] * k * pairs, ms)
+        meth[m] = rec
+    out["methods_8192x256_fp32"] = meth
+
+    # batched: 4096 sets, log-uniform sizes in [32, 2048], d in {16, 32, 64, 128}
+    from gputools_amd.autotune import tune_batched
+    from gputools_amd.batched import Batch
+    sizes, dims = wl.batched_sets(4096, seed=5)
+    rng = np.random.default_rng(55)
+    xb = torch.from_numpy(rng.standard_normal(int((sizes * dims).sum()),
+                                              dtype=np.float32)).cuda()
+    bat = {}
+    for m, fpf in (("euclidean", 3), ("manhattan", 2)):
+        b = Batch(x=xb, sizes=sizes, dims=dims, method=m)
+        tile = tune_batched(b, reps=3)
+        g = b.capture(tile)
+        ms_graph = _timed(g.replay, steps, warmup, torch)
+        ms_launch = _timed(lambda: b.run(tile), steps, warmup, torch)
+        ms_per_set = _timed(b.run_per_set, 1, 1, torch)
+        flops = float(fpf * (sizes * (sizes - 1) // 2 * dims).sum())
+        bat[m] = {"tile": tile, "ms_graph_replay": round(ms_graph, 4),
+                  "ms_one_launch": round(ms_launch, 4), "ms_per_set_launches": round(ms_per_set, 4),
+                  "speedup_vs_per_set": round(ms_per_set / ms_graph, 2),
+                  "pairs_per_s": round(b.pairs / (ms_graph * 1e-3), 1),
+                  "roofline": _valu_roof(flops, ms_graph)}
+    out["batched_4096_sets"] = bat
+
+    # backward and MDS stress: delta from 16384 x 128 points
+    n, k = 16384, 128
+    pairs = n * (n - 1) // 2
+    x = torch.from_numpy(wl.normal(n, k, seed=4)).cuda()
+    delta = gt.dist(x)
+    grad = torch.ones_like(delta)
+    ms_bw = _timed(lambda: gt.dist_backward(x, grad, "euclidean"), steps, warmup, torch)
+    # per pair and owner: 3k (recompute) + 2k (accumulate); each pair has two owners
+    out["backward_euclidean_16384x128"] = {
+        "ms": round(ms_bw, 4), "roofline": _valu_roof(2.0 * pairs * 5 * k, ms_bw)}
+    mds = {}
+    for q in (2, 32):
+        y = torch.from_numpy(wl.normal(n, q, seed=44)).cuda()
+
+        def step():
+            yy = y.detach().requires_grad_(True)
+            gt.mds_stress(yy, delta).backward()
+        ms = _timed(step, steps, warmup, torch)
+        mds["q=%d" % q] = {"ms_forward_backward": round(ms, 4),
+                           "roofline": _valu_roof(2.0 * pairs * 5 * q, ms)}
+    out["mds_stress_16384"] = mds
+
+    # the reference's small case: 1000 x 64 uniform, euclidean, latency per call
+    x = torch.from_numpy(wl.uniform(1000, 64, seed=1)).cuda()
+    o = torch.empty(1000 * 999 // 2, dtype=torch.float32, device="cuda")
+    out["uniform_1000x64_euclidean_ms"] = round(
+        _timed(lambda: gt.dist(x, "euclidean", out=o), 20, 3, torch), 4)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -102,6 +207,13 @@ def main():
     from gputools_amd import workloads as wl
 
     lib = gt.lib()
+    if args.calibrate_copy:
+        a = torch.ones(512 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)       # reads 2 GiB, writes 2 GiB
+        torch.cuda.synchronize()
+        del a, b
     n, k = args.n, args.k
     npairs = n * (n - 1) // 2
     print("bench: generating %dx%d mixture" % (n, k), file=sys.stderr, flush=True)
@@ -110,7 +222,13 @@ def main():
     out = torch.empty(npairs, dtype=torch.float32, device="cuda")
 
     results = {}
-    for precision in (args.precision, "fp32") if args.precision != "fp32" else ("fp32",):
+    if args.precisions:
+        order = [p for p in args.precisions.split(",") if p]
+        if args.precision not in order:
+            order.insert(0, args.precision)
+    else:
+        order = [args.precision, "fp32"] if args.precision != "fp32" else ["fp32"]
+    for precision in order:
         ws_bytes = gt.workspace_size(n, k, "euclidean", precision)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda") if ws_bytes else None
         steps = args.steps if precision == args.precision else (args.fp32_steps or args.steps)
@@ -145,6 +263,20 @@ def main():
                 "achieved": round(tflops, 2), "peak": PEAK_TFLOPS[args.precision],
                 "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS[args.precision], 4)}
     roof["traffic"] = None
+    # HBM bytes per launch of the dominant kernel, from the FETCH_SIZE /
+    # WRITE_SIZE counter runs recorded under profiles/ (scaled by the same
+    # counters' reading of a 2 GiB copy); null when no record matches.
+    tpath = os.path.join(ROOT, "profiles", "traffic_mixture_65536x512.json")
+    try:
+        with open(tpath) as f:
+            rec = json.load(f).get(args.precision)
+        if rec and n == N_POINTS and k == N_FEATURES:
+            roof["traffic"] = rec["hbm_bytes_per_launch"]
+            roof["traffic_source"] = "profiles/traffic_mixture_65536x512.json"
+            roof["traffic_over_algorithmic"] = round(rec["hbm_bytes_per_launch"] /
+                                                     bytes_per_launch, 2)
+    except (OSError, ValueError, KeyError):
+        pass
     roof["kernel_ms"] = round(ms, 4)
     roof["algorithmic_bytes"] = bytes_per_launch
     roof["algorithmic_flops"] = flops_per_launch
@@ -167,11 +299,18 @@ def main():
                    "precision": args.precision, "pairs": npairs},
         "roofline": roof,
     }
-    if args.precision != "fp32":
+    if args.precision != "fp32" and "fp32" in results:
         f = results["fp32"]
         line["fp32_plain"] = {"ms_per_step": round(f["ms_per_step"], 4),
                               "pairs_per_s": round(f["pairs_per_s"], 1)}
         line["speedup_vs_fp32"] = round(f["ms_per_step"] / ms, 3)
+    for other, rec in results.items():
+        if other not in (args.precision, "fp32"):
+            line.setdefault("other_precisions", {})[other] = {
+                "ms_per_step": round(rec["ms_per_step"], 4),
+                "pairs_per_s": round(rec["pairs_per_s"], 1),
+                "speedup_vs_fp32": (round(results["fp32"]["ms_per_step"] / rec["ms_per_step"], 3)
+                                    if "fp32" in results else None)}
 
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
@@ -195,6 +334,10 @@ def main():
         line["max_rel_err"] = errs[args.precision]
         line["max_rel_err_by_path"] = {p: float("%.3g" % e) for p, e in errs.items()}
         line["error_sample_pairs"] = int(ref.size)
+
+    if not args.no_paths:
+        print("bench: secondary paths", file=sys.stderr, flush=True)
+        line["paths"] = secondary_paths(torch, gt, wl)
 
     print(json.dumps(line), flush=True)
 

@@ -48,6 +48,7 @@ c_cpp = ctypes.POINTER(ctypes.c_char_p)
 # name -> (restype, argtypes)
 SIGNATURES = {
     "Rdistances": (None, [c_fp, c_ip, c_ip, c_fp, c_cpp, c_fp]),
+    "Rdistances_packed": (None, [c_fp, c_ip, c_ip, c_fp, c_cpp, c_fp]),
     "Rdistclust": (None, [c_cpp, c_cpp, c_fp, c_ip, c_ip, c_ip, c_ip, c_fp]),
     "cuCompile": (None, [c_ip, c_cpp, c_cpp]),
     "unloadPackage": (None, []),

@@ -420,6 +420,20 @@ void Rdistances(const float *points, const int *numPoints, const int *dim, float
     raise_in_r(s);
 }
 
+void Rdistances_packed(const float *points, const int *numPoints, const int *dim,
+                       float *distances, const char **method, const float *p)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!numPoints || !dim || !method || !p) {
+        s = fail(GT_ERR_INVALID, "Rdistances_packed: null argument");
+    } else {
+        const int m = dist_method_from_name(*method);
+        s = dist_host(points, *numPoints, *dim, m, *p, GT_PREC_FP32, GT_LAYOUT_PACKED, distances);
+    }
+    raise_in_r(s);
+}
+
 void Rdistclust(const char **distmethod, const char **clustmethod, const float *points,
                 const int *numPoints, const int *dim, int *merge, int *order, float *val)
 {

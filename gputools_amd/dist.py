@@ -164,6 +164,24 @@ def Rdistances(points: np.ndarray, method: str, p: float) -> np.ndarray:
     return d.reshape(n, n)
 
 
+def Rdistances_packed(points: np.ndarray, method: str, p: float) -> np.ndarray:
+    """The packed .C export, called as an R shim would: host pointers, a
+    caller-allocated n(n-1)/2 output in R 'dist' order."""
+    x = np.ascontiguousarray(points, dtype=np.float32)
+    n, k = x.shape
+    d = np.zeros(max(n * (n - 1) // 2, 1), dtype=np.float32)
+    cn, ck = ctypes.c_int(n), ctypes.c_int(k)
+    cm = (ctypes.c_char_p * 1)(method.encode())
+    cp = ctypes.c_float(p)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib().Rdistances_packed(x.ctypes.data_as(fp), ctypes.byref(cn), ctypes.byref(ck),
+                            d.ctypes.data_as(fp), cm, ctypes.byref(cp))
+    err = _lib.last_error()
+    if err:
+        raise _lib.GputoolsError(_lib.GT_ERR_HIP, err)
+    return d[:n * (n - 1) // 2]
+
+
 # --------------------------------------------------------------------------
 # Device API (torch tensors in HBM)
 # --------------------------------------------------------------------------

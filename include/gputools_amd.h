@@ -86,6 +86,14 @@ enum gt_status {
 void Rdistances(const float *points, const int *numPoints, const int *dim,
                 float *distances, const char **method, const float *p);
 
+/* Packed variant of Rdistances for R's .C: writes the n(n-1)/2 values of the
+ * 'dist' object (lower triangle, column order) into 'distances' directly, so
+ * the R side needs neither single(n * n) (which overflows R's integers for
+ * n >= 46341, R/gpuHclust.R:25) nor as.dist (R/gpuHclust.R:29).  No
+ * counterpart in the reference; same argument conventions as Rdistances. */
+void Rdistances_packed(const float *points, const int *numPoints, const int *dim,
+                       float *distances, const char **method, const float *p);
+
 /* Replaces Rdistclust (src/rinterface.cpp:172-204, decl src/rinterface.h:50-52).
  * The distance stage runs (device-full layout, p = 2 as at
  * src/rinterface.cpp:182); the hierarchical-clustering stage

@@ -176,6 +176,14 @@ def test_Rdistances_dotC_convention():
         assert_parity(full[ii[order], jj[order]], want, eff)
 
 
+def test_Rdistances_packed_dotC_convention():
+    x = wl.normal(300, 17, seed=22)
+    for method, p in (("canberra", 2.0), ("binary", 2.0), ("minkowski", 3.0), ("manhattan", 2.0)):
+        xin = np.abs(x) + 0.1 if method == "canberra" else x
+        got = gt.Rdistances_packed(xin, method, p)
+        assert_parity(got, orc.dist(xin, method, p), method)
+
+
 def test_gpuDist_mirror():
     x = wl.uniform(50, 10, seed=4)
     d = gt.gpuDist(x, "minkowski", 4)
