@@ -46,7 +46,7 @@ constexpr int GBM = 256;        // tile side (points)
 constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
 constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
-constexpr int GTHREADS = 512;   // 8 waves in a 4 (A) x 2 (B) grid, each 64 x 128 pairs
+constexpr int GTHREADS = 1024;  // 16 waves in a 4 (A) x 4 (B) grid, each 64 x 64 pairs
 constexpr int GSUPER = 2;       // super-tile side in tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
 constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     }
 }
 
-// Exclusive scan of one int per thread over the 256-thread block.
+// Exclusive scan of one int per thread over the GTHREADS-thread block.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;   // A rows wr*64.., B rows wc*128..
+    const int wr = wave >> 2, wc = wave & 3;   // A rows wr*64.., B rows wc*64..
 
     // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
     // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         const int64_t r = j0 + tid;
         nA[tid] = r < n ? args.norms[r] : 0.0f;
         if (SPLIT == SPLIT_FP16) iA[tid] = r < n ? args.iscale[r] : 1.0f;
-    } else {
+    } else if (tid < 2 * GBM) {
         const int64_t r = i0 + (tid - GBM);
         nB[tid - GBM] = r < n ? args.norms[r] : 0.0f;
         if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < n ? args.iscale[r] : 1.0f;
@@ -230,45 +230,41 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     // swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
     // 8 or 16 consecutive rows hit distinct banks without padding.
     // global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so
-    // each wave instruction fills 32 rows: wave w fills rows 32w..32w+31 of
-    // every (side, term) block.  Rows past n are clamped to row n-1: they only
-    // feed pairs that are never stored.
+    // each wave instruction fills 32 rows: waves 0-7 fill side A and waves
+    // 8-15 side B, wave w rows 32(w%8)..32(w%8)+31 of every term block of its
+    // side.  Rows past n are clamped to row n-1: they only feed pairs that are
+    // never stored.
     typedef __attribute__((address_space(3))) void lds_void_t;
     typedef const __attribute__((address_space(1))) void gbl_void_t;
     const int64_t term_stride = n * kp;
-    const unsigned short *gsA, *gsB;
+    const int lside = wave >> 3, lrb = wave & 7;
+    const unsigned short *gsrc;
     {
-        const int row = wave * 32 + (lane >> 1);
+        const int row = lrb * 32 + (lane >> 1);
         const int half = (lane & 1) ^ (((row >> 2) ^ (row >> 3)) & 1);
-        int64_t r = j0 + row;
-        gsA = args.terms + (r < n ? r : n - 1) * kp + half * 8;
-        r = i0 + row;
-        gsB = args.terms + (r < n ? r : n - 1) * kp + half * 8;
+        const int64_t r = (lside ? i0 : j0) + row;
+        gsrc = args.terms + (r < n ? r : n - 1) * kp + half * 8;
     }
-    unsigned char *const lbase = smem + wave * 1024;   // + block/stage offsets
+    unsigned char *const lbase = smem + lrb * 1024;   // + block/stage offsets
     auto issue = [&](int chunk) {
         const int g = chunk % L::NSTAGE;
         const int64_t kb = (int64_t)chunk * GBK;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+        for (int t = 0; t < NT; ++t)
             __builtin_amdgcn_global_load_lds(
-                (gbl_void_t *)(gsA + t * term_stride + kb),
-                (lds_void_t *)(lbase + g * L::STAGE + (0 * NT + t) * TERM_BYTES), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(
-                (gbl_void_t *)(gsB + t * term_stride + kb),
-                (lds_void_t *)(lbase + g * L::STAGE + (1 * NT + t) * TERM_BYTES), 16, 0, 0);
-        }
+                (gbl_void_t *)(gsrc + t * term_stride + kb),
+                (lds_void_t *)(lbase + g * L::STAGE + (lside * NT + t) * TERM_BYTES), 16, 0, 0);
     };
     // operand read offset of this lane (rows differ by multiples of 32 between
     // blocks, which leaves swz unchanged)
     const int lrow = lane & 31;
     const int roff = lrow * GROWB + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
 
-    f32x16_t acc[2][4];
+    f32x16_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
@@ -278,13 +274,13 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     for (int c = 0; c < L::NSTAGE - 1; ++c)
         if (c < nchunks) issue(c);
     for (int c = 0; c < nchunks; ++c) {
-        // chunk c landed when at most the later chunks' 2*NT loads per wave
-        // are outstanding (vector memory loads complete in order)
+        // chunk c landed when at most the next chunk's NT loads per wave are
+        // outstanding (vector memory loads complete in order)
         if (c + 1 < nchunks) {
             if (NT == 3)
-                asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+                asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
             else
-                asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+                asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
         } else {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
@@ -303,12 +299,12 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
                 av[t][a] = *reinterpret_cast<const vec *>(sg + t * TERM_BYTES +
                                                           (wr * 64 + a * 32) * GROWB);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < 2; ++b) {
             vec bv[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 bv[t] = *reinterpret_cast<const vec *>(sg + (NT + t) * TERM_BYTES +
-                                                       (wc * 128 + b * 32) * GROWB);
+                                                       (wc * 64 + b * 32) * GROWB);
             // hh' first, then the cross terms in decreasing magnitude
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
@@ -331,15 +327,16 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     // r%4 (A side, j) and column l%32 (B side, i); lanes 0..31 hold 32
     // consecutive i of one j, so packed stores are contiguous runs.
     const bool interior = (I > J) && (i0 + GBM <= n);   // every pair valid, i > j
-    const int li0 = wc * 128 + (lane & 31);
-    float ni[4], sb[4];
+    const int li0 = wc * 64 + (lane & 31);
+    float ni[2], sb[2];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < 2; ++b) {
         ni[b] = nB[li0 + b * 32];
         sb[b] = (SPLIT == SPLIT_FP16) ? iB[li0 + b * 32] : 1.0f;
     }
     const float tau = args.tau;
-    unsigned long long fmask[2] = {0ull, 0ull};
+    // flag bit (2a + b) * 16 + r marks accumulator element r of block (a, b)
+    unsigned long long fmask = 0ull;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -351,7 +348,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
             float *rowp = args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd) +
                           i0 + li0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < 2; ++b) {
                 float g = acc[a][b][r];
                 // undo the row scales one at a time (their product can overflow)
                 if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
@@ -366,7 +363,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
                         rowp[b * 32] = v;
                         if (LAYOUT == OUT_FULL) args.out[(i0 + li) * args.ldd + j] = v;
                     } else {
-                        fmask[a] |= 1ull << (b * 16 + r);
+                        fmask |= 1ull << ((a * 2 + b) * 16 + r);
                     }
                 }
                 if (LAYOUT == OUT_FULL && !interior && i0 + li == j && j < n) rowp[b * 32] = 0.0f;
@@ -375,14 +372,14 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
 
     // ---- exact FP32 recompute of the flagged pairs -------------------------
     int total;
-    const int myoff = block_exclusive_scan(__popcll(fmask[0]) + __popcll(fmask[1]), scan_tmp, total);
+    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
     if (total == 0) return;
     const int64_t ldx = args.ldx;
-    // Staging unit e = tid + 512 q (q < FUNITS): side e >> 11, row
+    // Staging unit e = tid + 1024 q (q < FUNITS): side e >> 11, row
     // (e >> 3) & 255, features (e & 7) * 4..+3 of a 32-feature chunk.  The
     // accumulators are dead here, so their registers hold a ring of FRING
     // chunks in flight (the pass is bound by load latency, not bandwidth).
-    constexpr int FRING = 4;
+    constexpr int FRING = 2;
     const float *fsrc[FUNITS];
     int fcol[FUNITS];
 #pragma unroll
@@ -394,7 +391,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         fcol[q] = (e & 7) * 4;
     }
     const int nf = (int)((k + FBK - 1) / FBK);
-    float4 fr0[FUNITS], fr1[FUNITS], fr2[FUNITS], fr3[FUNITS];
+    float4 fr0[FUNITS], fr1[FUNITS];
 #define GT_FLOAD(buf, chunk)                                                               \
     do {                                                                                   \
         const int64_t fb_ = (int64_t)(chunk) * FBK;                                        \
@@ -447,20 +444,17 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     for (int base = 0; base < total; base += FCAP) {
         {
             int rank = myoff;
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                unsigned long long m = fmask[a];
-                while (m) {
-                    const int bit = __builtin_ctzll(m);
-                    m &= m - 1;
-                    if (rank >= base && rank < base + FCAP) {
-                        const int b = bit >> 4, r = bit & 15;
-                        const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        const int li = li0 + b * 32;
-                        list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
-                    }
-                    ++rank;
+            unsigned long long m = fmask;
+            while (m) {
+                const int bit = __builtin_ctzll(m);
+                m &= m - 1;
+                if (rank >= base && rank < base + FCAP) {
+                    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
+                    const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int li = li0 + b * 32;
+                    list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
                 }
+                ++rank;
             }
         }
         const int cnt = (total - base) < FCAP ? (total - base) : FCAP;
@@ -470,8 +464,6 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         for (int c = 0; c < FPER; ++c) sum[c] = 0.0f;
         GT_FLOAD(fr0, 0);
         if (nf > 1) GT_FLOAD(fr1, 1);
-        if (nf > 2) GT_FLOAD(fr2, 2);
-        if (nf > 3) GT_FLOAD(fr3, 3);
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
@@ -481,8 +473,6 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         for (int ch = 0; ch < nf; ch += FRING) {
             GT_FSTEP(fr0, ch);
             if (ch + 1 < nf) GT_FSTEP(fr1, ch + 1);
-            if (ch + 2 < nf) GT_FSTEP(fr2, ch + 2);
-            if (ch + 3 < nf) GT_FSTEP(fr3, ch + 3);
         }
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
