@@ -123,6 +123,10 @@ int check_dist_args(const float *x, int64_t n, int64_t k, int64_t ldx, int metho
     if (precision != GT_PREC_FP32 && method != GT_EUCLIDEAN)
         return fail(GT_ERR_UNSUPPORTED,
                     "split-precision Gram form exists for euclidean only (method %d)", method);
+    // the Gram epilogue addresses a wave's 64 output rows with 32-bit offsets
+    // (an output of that size could not be allocated anyway)
+    if (precision != GT_PREC_FP32 && (n >= (int64_t(1) << 24) || (layout == GT_LAYOUT_FULL && ldd >= (int64_t(1) << 24))))
+        return fail(GT_ERR_INVALID, "split-precision path supports n and ldd below 2^24");
     const bool has_out = (layout == GT_LAYOUT_PACKED) ? n > 1 : n > 0;
     if (has_out && out == nullptr) return fail(GT_ERR_INVALID, "null output pointer");
     if (has_out && k > 0 && x == nullptr) return fail(GT_ERR_INVALID, "null input pointer");

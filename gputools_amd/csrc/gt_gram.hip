@@ -51,6 +51,10 @@ constexpr int GSUPER = 2;       // super-tile side in tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
 constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
 constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
+// Gram-form values are finished with the hardware square root (v_sqrt_f32)
+// plus one residual correction (sqrt_rn), which rounds correctly for normal
+// inputs; d^2 below 2^-100 goes to the exact pass instead.
+constexpr float GRAM_SQRT_MIN = 0x1p-100f;
 constexpr int FPER = FCAP / GTHREADS;
 constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
 
@@ -159,6 +163,21 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     }
 }
 
+// Correctly rounded square root for x >= 2^-100 (normal) or +inf.  The
+// hardware v_sqrt_f32 result s is within 1 ulp; the residuals x - s' s of
+// its neighbours s' = s -/+ 1 ulp, each exact in one fma, tell whether the
+// root lies below s - ulp/2 or above s + ulp/2.  For x = +inf both residuals
+// are NaN and s = +inf stays; for x = 0 the result is 0.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r = fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
+}
+
 // Exclusive scan of one int per thread over the GTHREADS-thread block.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
 {
@@ -198,7 +217,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     float *iB = iA + GBM;
     int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 2, wc = wave & 3;   // A rows wr*64.., B rows wc*64..
 
     // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
@@ -337,43 +356,81 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     const float tau = args.tau;
     // flag bit (2a + b) * 16 + r marks accumulator element r of block (a, b)
     unsigned long long fmask = 0ull;
+    if (interior) {
+        // rows jw + d (d < 64) of this wave: element offsets from the start of
+        // row jw fit in 32 bits (the host limits n to below 2^24), and the
+        // row-dependent part of each offset is wave-uniform
+        const int64_t jw = j0 + wr * 64;
+        float *const outw =
+            args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, jw) : jw * args.ldd) + i0;
+        const unsigned cw = (LAYOUT == OUT_PACKED) ? (unsigned)(n - 1 - jw) : (unsigned)args.ldd;
+        const bool hi = (lane >> 5) != 0;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int64_t j = j0 + lj;
-            const float nj = nA[lj];
-            const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
-            float *rowp = args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd) +
-                          i0 + li0;
+            for (int r = 0; r < 16; ++r) {
+                const unsigned dc = a * 32 + 8 * (r >> 2) + (r & 3);
+                const unsigned base_u = (LAYOUT == OUT_PACKED) ? dc * cw - dc * (dc + 1) / 2 : dc * cw;
+                const unsigned step_u = (LAYOUT == OUT_PACKED) ? 4 * cw - 4 * dc - 10 : 4 * cw;
+                const unsigned roff = base_u + (hi ? step_u : 0u) + (unsigned)li0;
+                const int lj = wr * 64 + (int)dc + (hi ? 4 : 0);
+                const float nj = nA[lj];
+                const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                float g = acc[a][b][r];
-                // undo the row scales one at a time (their product can overflow)
-                if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
-                const float ss = nj + ni[b];
-                const float d2 = fmaf(-2.0f, g, ss);
-                const bool flag = !(d2 >= tau * ss);
-                const int li = li0 + b * 32;
-                const bool valid = interior || (i0 + li < n && i0 + li > j);
-                if (valid) {
-                    if (!flag) {
-                        const float v = sqrtf(fmaxf(d2, 0.0f));
-                        rowp[b * 32] = v;
-                        if (LAYOUT == OUT_FULL) args.out[(i0 + li) * args.ldd + j] = v;
-                    } else {
-                        fmask |= 1ull << ((a * 2 + b) * 16 + r);
-                    }
+                for (int b = 0; b < 2; ++b) {
+                    float g = acc[a][b][r];
+                    if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
+                    const float ss = nj + ni[b];
+                    const float d2 = fmaf(-2.0f, g, ss);
+                    const bool ok = d2 >= fmaf(tau, ss, GRAM_SQRT_MIN);
+                    fmask |= ok ? 0ull : (1ull << ((a * 2 + b) * 16 + r));
+                    const float v = sqrt_rn(d2);
+                    outw[roff + (unsigned)(b * 32)] = v;
+                    if (LAYOUT == OUT_FULL) args.out[(i0 + li0 + b * 32) * args.ldd + jw + lj - wr * 64] = v;
                 }
-                if (LAYOUT == OUT_FULL && !interior && i0 + li == j && j < n) rowp[b * 32] = 0.0f;
             }
-        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t j = j0 + lj;
+                const float nj = nA[lj];
+                const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
+                float *rowp = args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd) +
+                              i0 + li0;
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    float g = acc[a][b][r];
+                    // undo the row scales one at a time (their product can overflow)
+                    if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
+                    const float ss = nj + ni[b];
+                    const float d2 = fmaf(-2.0f, g, ss);
+                    const bool flag = !(d2 >= fmaxf(tau * ss, GRAM_SQRT_MIN));
+                    const int li = li0 + b * 32;
+                    const bool valid = i0 + li < n && i0 + li > j;
+                    if (valid) {
+                        if (!flag) {
+                            const float v = sqrt_rn(fmaxf(d2, 0.0f));
+                            rowp[b * 32] = v;
+                            if (LAYOUT == OUT_FULL) args.out[(i0 + li) * args.ldd + j] = v;
+                        } else {
+                            fmask |= 1ull << ((a * 2 + b) * 16 + r);
+                        }
+                    }
+                    if (LAYOUT == OUT_FULL && i0 + li == j && j < n) rowp[b * 32] = 0.0f;
+                }
+            }
+    }
 
     // ---- exact FP32 recompute of the flagged pairs -------------------------
     int total;
     const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
     if (total == 0) return;
+    // the epilogue's stores of flagged pairs land before the exact values
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
     const int64_t ldx = args.ldx;
     // Staging unit e = tid + 1024 q (q < FUNITS): side e >> 11, row
     // (e >> 3) & 255, features (e & 7) * 4..+3 of a 32-feature chunk.  The
