@@ -18,6 +18,9 @@ METHOD_NAMES = ("euclidean", "maximum", "manhattan", "canberra", "binary", "mink
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16x3": PREC_FP16X3}
 LAYOUT_PACKED, LAYOUT_FULL = 0, 1
+# gt_hc_method (hc_method order, src/hcluster.h:5-16); names as getClusterEnum spells them
+HC_METHOD_NAMES = ("single", "complete", "wpgma", "average", "median", "centroid",
+                   "flexible_group", "flexible", "ward", "mcquitty")
 GT_OK, GT_ERR_INVALID, GT_ERR_HIP, GT_ERR_ALLOC, GT_ERR_UNSUPPORTED, GT_ERR_WORKSPACE = range(6)
 STATUS_NAMES = {0: "GT_OK", 1: "GT_ERR_INVALID", 2: "GT_ERR_HIP", 3: "GT_ERR_ALLOC",
                 4: "GT_ERR_UNSUPPORTED", 5: "GT_ERR_WORKSPACE"}
@@ -74,6 +77,13 @@ SIGNATURES = {
     "gt_dist_batched_schedule": (c_int, [c_p, c_p, c_i64, c_int, c_p, c_i64,
                                          ctypes.POINTER(c_i64)]),
     "gt_batched_tile_count": (c_int, []),
+    "Rhcluster": (None, [c_fp, c_ip, c_ip, c_ip, c_fp, c_cpp]),
+    "gt_hclust_method_from_name": (c_int, [ctypes.c_char_p]),
+    "gt_hclust_workspace_size": (c_int, [c_i64, ctypes.POINTER(c_size)]),
+    "gt_hclust": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p, c_p, c_size,
+                          c_p]),
+    "gt_hclust_host": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p]),
+    "gt_hclust_format": (c_int, [c_i64, c_p, c_p, c_p, c_p]),
 }
 
 

@@ -265,6 +265,123 @@ int dist_host(const float *x_host, int64_t n, int64_t k, int method, float p, in
     return GT_OK;
 }
 
+// ------------------------------ clustering ---------------------------------
+
+int hclust_method_from_name(const char *name)
+{
+    // getClusterEnum (src/rinterface.cpp:158-170): exact names, anything else
+    // (including R's "flexible group", which gpuHclust passes) -> single.
+    if (name == nullptr) return GT_HC_SINGLE;
+    if (strcmp(name, "complete") == 0) return GT_HC_COMPLETE;
+    if (strcmp(name, "wpgma") == 0) return GT_HC_WPGMA;
+    if (strcmp(name, "average") == 0) return GT_HC_AVERAGE;
+    if (strcmp(name, "median") == 0) return GT_HC_MEDIAN;
+    if (strcmp(name, "centroid") == 0) return GT_HC_CENTROID;
+    if (strcmp(name, "flexible_group") == 0) return GT_HC_FLEXIBLE_GROUP;
+    if (strcmp(name, "flexible") == 0) return GT_HC_FLEXIBLE;
+    if (strcmp(name, "ward") == 0) return GT_HC_WARD;
+    if (strcmp(name, "mcquitty") == 0) return GT_HC_MCQUITTY;
+    return GT_HC_SINGLE;
+}
+
+size_t hclust_ws_bytes(int64_t n)
+{
+    return 3 * round_up((size_t)(n > 0 ? n : 0) * 4, 256);
+}
+
+int check_hclust_args(int64_t n, int64_t ldd, int method)
+{
+    if (n < 0) return fail(GT_ERR_INVALID, "negative size n=%lld", (long long)n);
+    if (n >= (int64_t(1) << 24))
+        return fail(GT_ERR_INVALID, "clustering supports n below 2^24 (n=%lld)", (long long)n);
+    if (ldd < n) return fail(GT_ERR_INVALID, "ldd=%lld < n=%lld", (long long)ldd, (long long)n);
+    if (method < GT_HC_SINGLE || method > GT_HC_MCQUITTY)
+        return fail(GT_ERR_INVALID, "unknown clustering method %d", method);
+    return GT_OK;
+}
+
+int run_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta, int *sub,
+               int *sup, float *height, void *ws, hipStream_t stream)
+{
+    char *w = static_cast<char *>(ws);
+    const size_t part = round_up((size_t)n * 4, 256);
+    GT_HIP(gt::launch_hclust(d, n, ldd, method, lambda, beta, reinterpret_cast<float *>(w),
+                             reinterpret_cast<float *>(w + part), reinterpret_cast<int *>(w + 2 * part),
+                             sub, sup, height, stream));
+    return GT_OK;
+}
+
+// Merges and heights from a device-resident full matrix (consumed) to host.
+int hclust_device_to_host(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                          int *sub_h, int *sup_h, float *height_h)
+{
+    if (n < 2) return GT_OK;
+    const size_t len = (size_t)(n - 1);
+    DeviceBuffer ws, out;
+    int s;
+    if ((s = ws.alloc(hclust_ws_bytes(n), "hipMalloc(clustering workspace)")) != GT_OK) return s;
+    if ((s = out.alloc(3 * round_up(len * 4, 256), "hipMalloc(clustering result)")) != GT_OK) return s;
+    char *o = static_cast<char *>(out.ptr);
+    const size_t part = round_up(len * 4, 256);
+    int *dsub = reinterpret_cast<int *>(o);
+    int *dsup = reinterpret_cast<int *>(o + part);
+    float *dh = reinterpret_cast<float *>(o + 2 * part);
+    if ((s = run_hclust(d, n, ldd, method, lambda, beta, dsub, dsup, dh, ws.ptr, nullptr)) != GT_OK)
+        return s;
+    GT_HIP(hipMemcpy(sub_h, dsub, len * 4, hipMemcpyDeviceToHost));
+    GT_HIP(hipMemcpy(sup_h, dsup, len * 4, hipMemcpyDeviceToHost));
+    GT_HIP(hipMemcpy(height_h, dh, len * 4, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+// R's hclust merge matrix and order from the merge sequence, as
+// formatClustering / getPrintOrder / depthFirst (src/rinterface.cpp:241-292)
+// define them.  The reference rewrites every later reference to a merged
+// point in an O(len^2) sweep; the same result in O(len): a point that
+// survived step i stands for cluster i + 1 from then on.  Rows list a
+// singleton before a cluster and two clusters in increasing order.  The
+// order is the leaves of a depth-first walk from the last merge that emits
+// the right subtree's leaves from the end of the array backwards first.
+int hclust_format(int64_t len, const int *sub, const int *sup, int *merge, int *order)
+{
+    if (len < 0) return fail(GT_ERR_INVALID, "negative length");
+    if (len == 0) {
+        if (order) order[0] = 1;
+        return GT_OK;
+    }
+    const int64_t n = len + 1;
+    std::vector<int> id((size_t)n, 0);
+    for (int64_t i = 0; i < len; ++i) {
+        const int p = sub[i], q = sup[i];
+        if (p < 0 || p >= n || q < 0 || q >= n || p == q)
+            return fail(GT_ERR_INVALID, "invalid merge %lld: (%d, %d)", (long long)i, p, q);
+        int a = id[p] ? id[p] : -(p + 1);
+        int b = id[q] ? id[q] : -(q + 1);
+        if ((b < 0 && a > 0) || (a > 0 && b > 0 && a > b)) std::swap(a, b);
+        merge[i] = a;
+        merge[i + len] = b;
+        id[q] = (int)(i + 1);
+        id[p] = (int)(i + 1);
+    }
+    std::vector<int> stack;
+    stack.reserve((size_t)n);
+    stack.push_back((int)len);   // cluster ids are 1-based
+    int64_t otop = len;
+    while (!stack.empty()) {
+        const int node = stack.back();
+        stack.pop_back();
+        if (node < 0) {
+            if (otop < 0) return fail(GT_ERR_INVALID, "merge sequence is not a tree");
+            order[otop--] = -node;
+            continue;
+        }
+        // right subtree first (it fills the end of the array), then left
+        stack.push_back(merge[node - 1]);
+        stack.push_back(merge[node - 1 + len]);
+    }
+    return GT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,6 +525,53 @@ int gt_free(void *device_ptr)
     return GT_OK;
 }
 
+int gt_hclust_method_from_name(const char *name) { return hclust_method_from_name(name); }
+
+int gt_hclust_workspace_size(int64_t n, size_t *bytes)
+{
+    if (bytes == nullptr) return fail(GT_ERR_INVALID, "null bytes pointer");
+    if (n < 0) return fail(GT_ERR_INVALID, "negative size");
+    *bytes = hclust_ws_bytes(n);
+    return GT_OK;
+}
+
+int gt_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta, int *sub,
+              int *sup, float *height, void *workspace, size_t workspace_bytes, void *stream)
+{
+    int s = check_hclust_args(n, ldd, method);
+    if (s != GT_OK) return s;
+    if (n < 2) return GT_OK;
+    if (!d || !sub || !sup || !height) return fail(GT_ERR_INVALID, "null pointer");
+    if (workspace == nullptr || workspace_bytes < hclust_ws_bytes(n))
+        return fail(GT_ERR_WORKSPACE, "clustering workspace of %zu bytes needed (got %zu)",
+                    hclust_ws_bytes(n), workspace_bytes);
+    return run_hclust(d, n, ldd, method, lambda, beta, sub, sup, height, workspace,
+                      static_cast<hipStream_t>(stream));
+}
+
+int gt_hclust_host(const float *d_host, int64_t n, int64_t ldd, int method, float lambda,
+                   float beta, int *sub, int *sup, float *height)
+{
+    int s = check_hclust_args(n, ldd, method);
+    if (s != GT_OK) return s;
+    if (n < 2) return GT_OK;
+    if (!d_host || !sub || !sup || !height) return fail(GT_ERR_INVALID, "null pointer");
+    DeviceBuffer dd;
+    if ((s = dd.alloc((size_t)n * (size_t)n * sizeof(float), "hipMalloc(distances)")) != GT_OK)
+        return s;
+    GT_HIP(hipMemcpy2D(dd.ptr, (size_t)n * sizeof(float), d_host, (size_t)ldd * sizeof(float),
+                       (size_t)n * sizeof(float), (size_t)n, hipMemcpyHostToDevice));
+    return hclust_device_to_host(static_cast<float *>(dd.ptr), n, n, method, lambda, beta, sub, sup,
+                                 height);
+}
+
+int gt_hclust_format(int64_t len, const int *sub, const int *sup, int *merge, int *order)
+{
+    if (len < 0 || (len > 0 && (!sub || !sup || !merge)) || !order)
+        return fail(GT_ERR_INVALID, "null or negative argument");
+    return hclust_format(len, sub, sup, merge, order);
+}
+
 // --------------------------- .C drop-in exports ---------------------------
 
 void Rdistances(const float *points, const int *numPoints, const int *dim, float *distances,
@@ -442,24 +606,49 @@ void Rdistclust(const char **distmethod, const char **clustmethod, const float *
                 const int *numPoints, const int *dim, int *merge, int *order, float *val)
 {
     g_err[0] = '\0';
-    (void)clustmethod;
-    (void)merge;
-    (void)order;
-    (void)val;
     int s;
-    if (!distmethod || !numPoints || !dim) {
+    if (!distmethod || !clustmethod || !numPoints || !dim) {
         s = fail(GT_ERR_INVALID, "Rdistclust: null argument");
+    } else if (*numPoints < 1) {
+        s = fail(GT_ERR_INVALID, "Rdistclust: need at least one point");
     } else {
-        // Distance stage, as Rdistclust runs it (src/rinterface.cpp:182-183):
-        // device-resident full matrix, p fixed at 2.
+        // Distance stage as Rdistclust runs it (src/rinterface.cpp:182-183):
+        // a device-resident full matrix, p fixed at 2; then the clustering
+        // stage on that matrix and R's merge / order formatting.
+        const int64_t n = *numPoints;
         float *d = nullptr;
         size_t pitch = 0;
-        s = gt_dist_device_full(points, *numPoints, *dim, dist_method_from_name(*distmethod), 2.0f,
-                                &d, &pitch);
+        s = gt_dist_device_full(points, n, *dim, dist_method_from_name(*distmethod), 2.0f, &d, &pitch);
+        if (s == GT_OK) {
+            std::vector<int> sub((size_t)(n > 1 ? n - 1 : 0)), sup(sub.size());
+            s = check_hclust_args(n, (int64_t)(pitch / sizeof(float)), hclust_method_from_name(*clustmethod));
+            if (s == GT_OK)
+                s = hclust_device_to_host(d, n, (int64_t)(pitch / sizeof(float)),
+                                          hclust_method_from_name(*clustmethod), 0.5f, 0.5f, sub.data(),
+                                          sup.data(), val);
+            if (s == GT_OK) s = hclust_format(n - 1, sub.data(), sup.data(), merge, order);
+        }
         if (d) (void)hipFree(d);
-        if (s == GT_OK)
-            s = fail(GT_ERR_UNSUPPORTED,
-                     "Rdistclust: the hierarchical-clustering stage is not part of this library");
+    }
+    raise_in_r(s);
+}
+
+void Rhcluster(const float *distMat, const int *numPoints, int *merge, int *order, float *val,
+               const char **method)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!distMat || !numPoints || !merge || !order || !method) {
+        s = fail(GT_ERR_INVALID, "Rhcluster: null argument");
+    } else if (*numPoints < 1) {
+        s = fail(GT_ERR_INVALID, "Rhcluster: need at least one point");
+    } else {
+        const int64_t n = *numPoints;
+        std::vector<int> sub((size_t)(n > 1 ? n - 1 : 0)), sup(sub.size());
+        // lambda = beta = 0.5, as Rhcluster passes them (src/rinterface.cpp:226-227)
+        s = gt_hclust_host(distMat, n, n, hclust_method_from_name(*method), 0.5f, 0.5f, sub.data(),
+                           sup.data(), val);
+        if (s == GT_OK) s = hclust_format(n - 1, sub.data(), sup.data(), merge, order);
     }
     raise_in_r(s);
 }

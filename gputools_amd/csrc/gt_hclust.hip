@@ -1,0 +1,367 @@
+// gt_hclust.hip -- the clustering stage of gpuDistClust / gpuHclust.
+//
+// Agglomerative clustering on a full, symmetric n x n FP32 distance matrix
+// in HBM (the layout gt_dist_device_full produces), with the ten
+// Lance-Williams updates of the reference (inst/cuda/hcluster.cu:106-400) and
+// its merge rule (src/hcluster.cpp:40-201): each round merges the closest
+// pair (sub < sup) over the lower triangle, the surviving cluster keeps index
+// sup, and its row and column are rewritten with the method's formula.
+//
+// The reference rescans the whole lower triangle every round (find_min1,
+// n/1024 launches of n^2/2 reads in total per round) and pays three kernel
+// launches plus a module load per launch and round.  Here:
+//   - hclust_init_kernel (many workgroups, one wave per row) puts +inf on the
+//     diagonal, sets every count to 1 and computes each row's minimum over
+//     columns 0..r-1 (value and column);
+//   - hclust_merge_kernel (one persistent workgroup of 1024 threads) runs all
+//     n - 1 rounds.  Per round: the argmin over the cached row minima, the
+//     update of row and column sup (row and column sub become +inf), then the
+//     cached minima are repaired: row sup is rescanned; a row r > sup takes
+//     the new D[r][sup] if it beats its cached minimum, and is rescanned only
+//     when that minimum sat at column sub or sup and the new value does not
+//     settle it; a row sub < r < sup is rescanned only when its minimum sat
+//     at column sub.  Rows whose minimum is unaffected cost one comparison.
+// Ties are broken exactly as the reference's 32-lane reductions do
+// (hcluster.cu:27-57, 67-90): smallest value, then the lane (index mod 32)
+// that wins the reduction tree (bit-reversed lane order), then smallest
+// index; +inf and NaN entries are never candidates.  The update
+// formulas keep the reference's arithmetic types (FP32, with the FP64
+// promotions its double constants cause for median, flexible and flexible
+// group) and round every product separately (no FMA contraction), so merges and heights
+// match the reference's loop bit for bit.  When no finite distance is left
+// (undefined in the reference) the two lowest-index clusters merge at +inf.
+#include "gt_common.h"
+#include "gt_internal.h"
+
+namespace gt {
+namespace {
+
+constexpr int HTHREADS = 1024;
+constexpr int HWAVES = HTHREADS / 64;
+constexpr int RESCAN_CAP = 2048;   // rows listed for rescan per round (LDS)
+constexpr int NO_COL = -1;         // row without a candidate
+constexpr int PENDING = -2;        // row queued for a rescan
+
+struct HclustArgs {
+    float *d;
+    int64_t ldd;
+    int n;
+    int method;
+    float lambda, beta;
+    float *count;      // > 0: cluster size; < 0: merged away (minus its size)
+    float *minv;       // cached row minimum over columns 0..r-1
+    int *minc;         // its column, NO_COL or PENDING
+    int *sub, *sup;
+    float *height;
+};
+
+// Order of equal values in the reference's 32-lane reductions: lane t keeps
+// the first of its indices t, t+32, ...; the tree then folds lanes t and
+// t + 16, t + 8, ..., t + 1 into t, keeping t on ties, so lane priority is
+// the 5-bit reversal of the lane number (even lanes before odd ones, ...).
+__device__ __forceinline__ unsigned tie_key(int idx)
+{
+    return ((__builtin_bitreverse32((unsigned)idx) >> 27) << 26) | ((unsigned)idx >> 5);
+}
+
+// (v, i) beats (bv, bi); bv = +inf with bi = NO_COL is the empty candidate
+__device__ __forceinline__ bool beats(float v, int i, float bv, int bi)
+{
+    return v < bv || (v == bv && bi != NO_COL && tie_key(i) < tie_key(bi));
+}
+
+// wave-wide argmin of (value, index); lanes without a candidate hold
+// (+inf, NO_COL)
+__device__ __forceinline__ void wave_argmin(float &v, int &i)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        if (oi != NO_COL && (i == NO_COL || beats(ov, oi, v, i))) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+
+// argmin of row r over columns 0..r-1 by one wave (all lanes return it)
+__device__ __forceinline__ void scan_row(const float *row, int r, float &bv, int &bi)
+{
+    const int lane = threadIdx.x & 63;
+    bv = INFINITY;
+    bi = NO_COL;
+    for (int c = lane; c < r; c += 64) {
+        const float v = row[c];
+        if (v < INFINITY && (bi == NO_COL || beats(v, c, bv, bi))) {
+            bv = v;
+            bi = c;
+        }
+    }
+    wave_argmin(bv, bi);
+}
+
+// A value the compiler cannot see through: a product passed through it is
+// rounded on its own and never fused into the following add (the library is
+// built with -ffp-contract=fast, under which contraction pragmas are ignored).
+__device__ __forceinline__ float sep(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ double sep(double x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Lance-Williams update of the reference's kernels (hcluster.cu:106-400):
+// bot = D[sup][c], top = D[sub][c]; nr = new size of sup, np = size of sub,
+// nq = old size of sup, nk = count[c], val = this round's merge height.
+// Every product is rounded separately, as the reference's unfused code does.
+__device__ __forceinline__ float lw_update(int method, float bot, float top, float nr, float np,
+                                           float nq, float nk, float val, float lambda, float beta)
+{
+    switch (method) {
+    case HC_SINGLE:
+        return fminf(bot, top);
+    case HC_COMPLETE:
+        return fmaxf(bot, top);
+    case HC_WPGMA:
+    case HC_MCQUITTY:
+        return (float)((double)(bot + top) / 2.0);
+    case HC_AVERAGE:
+        return (sep(top * np) + sep(bot * nq)) / nr;
+    case HC_MEDIAN:
+        return (float)((double)(bot + top) / 2.0 - (double)val / 4.0);
+    case HC_CENTROID:
+        return (sep(top * np) + sep(bot * nq)) / nr - (np * nq * val) / (nr * nr);
+    case HC_FLEXIBLE_GROUP:
+        return (float)((sep((double)bot * (1.0 - (double)lambda) * (double)np) +
+                        sep((double)top * (1.0 - (double)lambda) * (double)nq)) /
+                           (double)nr +
+                       (double)sep(beta * val));
+    case HC_FLEXIBLE:
+        return (float)((sep((double)bot * (1.0 - (double)lambda)) +
+                        sep((double)top * (1.0 - (double)lambda))) /
+                           2.0 +
+                       (double)sep(beta * val));
+    case HC_WARD: {
+        if (nr == -nk) return INFINITY;
+        float b = (sep(bot * (np + nk)) + sep(top * (nq + nk)) - sep(val * nk));
+        b /= (nr + nk);
+        return isinf(b) ? INFINITY : b;
+    }
+    default:
+        return fminf(bot, top);
+    }
+}
+
+// Diagonal to +inf, counts to 1, and every row's minimum: one wave per row.
+__global__ __launch_bounds__(256) void hclust_init_kernel(HclustArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < a.n; r += waves) {
+        float *row = a.d + r * a.ldd;
+        if (lane == 0) {
+            row[r] = INFINITY;
+            a.count[r] = 1.0f;
+        }
+        float bv;
+        int bi;
+        scan_row(row, (int)r, bv, bi);
+        if (lane == 0) {
+            a.minv[r] = bv;
+            a.minc[r] = bi;
+        }
+    }
+}
+
+__global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
+{
+    __shared__ float red_v[HWAVES];
+    __shared__ int red_i[HWAVES];
+    __shared__ int s_cw, s_rw, s_nres, s_overflow;
+    __shared__ float s_val;
+    __shared__ int rescan[RESCAN_CAP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = a.n;
+    const int64_t ldd = a.ldd;
+    for (int iter = 0; iter + 1 < n; ++iter) {
+        // ---- the closest pair: argmin over the cached row minima ----------
+        float bv = INFINITY;
+        int bi = NO_COL;
+        for (int r = tid; r < n; r += HTHREADS) {
+            const float v = a.minv[r];
+            if (a.minc[r] >= 0 && v < INFINITY && (bi == NO_COL || beats(v, r, bv, bi))) {
+                bv = v;
+                bi = r;
+            }
+        }
+        wave_argmin(bv, bi);
+        if (lane == 0) {
+            red_v[wave] = bv;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            float v = INFINITY;
+            int r = NO_COL;
+            for (int w = 0; w < HWAVES; ++w)
+                if (red_i[w] != NO_COL && (r == NO_COL || beats(red_v[w], red_i[w], v, r))) {
+                    v = red_v[w];
+                    r = red_i[w];
+                }
+            int cw, rw;
+            if (r != NO_COL) {
+                rw = r;
+                cw = a.minc[r];
+            } else {
+                // no finite distance left: the two lowest-index clusters
+                cw = -1;
+                rw = -1;
+                for (int i = 0; i < n && rw < 0; ++i)
+                    if (a.count[i] > 0.0f) {
+                        if (cw < 0)
+                            cw = i;
+                        else
+                            rw = i;
+                    }
+                v = INFINITY;
+            }
+            a.height[iter] = v;
+            a.sub[iter] = cw;
+            a.sup[iter] = rw;
+            a.count[rw] += a.count[cw];
+            a.count[cw] *= -1.0f;
+            s_cw = cw;
+            s_rw = rw;
+            s_val = v;
+            s_nres = 0;
+            s_overflow = 0;
+        }
+        __syncthreads();
+        const int cw = s_cw, rw = s_rw;
+        const float val = s_val;
+        const float nr = a.count[rw];
+        const float np = -a.count[cw];
+        const float nq = nr - np;
+
+        // ---- rewrite row/column sup; row/column sub become +inf ----------
+        float *const dcw = a.d + (int64_t)cw * ldd;
+        float *const drw = a.d + (int64_t)rw * ldd;
+        for (int c = tid; c < n; c += HTHREADS) {
+            float bot = INFINITY;
+            if (c != cw && c != rw)
+                bot = lw_update(a.method, drw[c], dcw[c], nr, np, nq, a.count[c], val, a.lambda,
+                                a.beta);
+            dcw[c] = INFINITY;
+            a.d[(int64_t)c * ldd + cw] = INFINITY;
+            drw[c] = bot;
+            a.d[(int64_t)c * ldd + rw] = bot;
+        }
+        __syncthreads();
+
+        // ---- repair the cached row minima --------------------------------
+        for (int r = cw + tid; r < n; r += HTHREADS) {
+            bool redo = false;
+            if (r == cw) {
+                a.minv[r] = INFINITY;
+                a.minc[r] = NO_COL;
+                continue;
+            }
+            if (a.count[r] <= 0.0f) continue;
+            const int mc = a.minc[r];
+            const float mv = a.minv[r];
+            if (r == rw) {
+                redo = true;
+            } else if (r < rw) {
+                redo = (mc == cw);
+            } else {
+                const float f = drw[r];   // the new D[r][sup]
+                if (mc == cw) {
+                    redo = !(f < mv);
+                    if (!redo) {
+                        a.minv[r] = f;
+                        a.minc[r] = rw;
+                    }
+                } else if (mc == rw) {
+                    redo = !(f <= mv);
+                    if (!redo) a.minv[r] = f;
+                } else if (f < INFINITY && (mc == NO_COL || beats(f, rw, mv, mc))) {
+                    a.minv[r] = f;
+                    a.minc[r] = rw;
+                }
+            }
+            if (redo) {
+                a.minc[r] = PENDING;
+                const int slot = atomicAdd(&s_nres, 1);
+                if (slot < RESCAN_CAP)
+                    rescan[slot] = r;
+                else
+                    s_overflow = 1;
+            }
+        }
+        __syncthreads();
+
+        // ---- rescans: one wave per row -----------------------------------
+        const int nres = s_nres < RESCAN_CAP ? s_nres : RESCAN_CAP;
+        for (int q = wave; q < nres; q += HWAVES) {
+            const int r = rescan[q];
+            float v;
+            int c;
+            scan_row(a.d + (int64_t)r * ldd, r, v, c);
+            if (lane == 0) {
+                a.minv[r] = v;
+                a.minc[r] = c;
+            }
+        }
+        if (s_overflow) {
+            // more rows than the list holds: find the rest by their marker
+            for (int r = cw + 1 + wave; r < n; r += HWAVES) {
+                if (a.minc[r] != PENDING) continue;
+                float v;
+                int c;
+                scan_row(a.d + (int64_t)r * ldd, r, v, c);
+                if (lane == 0) {
+                    a.minv[r] = v;
+                    a.minc[r] = c;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                         float *count, float *minv, int *minc, int *sub, int *sup, float *height,
+                         hipStream_t stream)
+{
+    if (n < 2) return hipSuccess;
+    HclustArgs a;
+    a.d = d;
+    a.ldd = ldd;
+    a.n = (int)n;
+    a.method = method;
+    a.lambda = lambda;
+    a.beta = beta;
+    a.count = count;
+    a.minv = minv;
+    a.minc = minc;
+    a.sub = sub;
+    a.sup = sup;
+    a.height = height;
+    int64_t blocks = (n + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(hclust_init_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(hclust_merge_kernel, dim3(1), dim3(HTHREADS), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace gt

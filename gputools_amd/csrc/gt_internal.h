@@ -68,4 +68,24 @@ hipError_t launch_batched(int op, int tile, const float *x, const int64_t *x_off
                           float *out, const int64_t *schedule, int64_t count, int64_t nsets,
                           int grid, hipStream_t stream);
 
+// Hierarchical clustering methods, in hc_method order (src/hcluster.h:5-16).
+enum HcMethod {
+    HC_SINGLE = 0,
+    HC_COMPLETE = 1,
+    HC_WPGMA = 2,
+    HC_AVERAGE = 3,
+    HC_MEDIAN = 4,
+    HC_CENTROID = 5,
+    HC_FLEXIBLE_GROUP = 6,
+    HC_FLEXIBLE = 7,
+    HC_WARD = 8,
+    HC_MCQUITTY = 9
+};
+
+// Clustering on a full n x n device matrix d (row stride ldd, destroyed).
+// count, minv, minc: n entries of workspace each; sub, sup, height: n - 1.
+hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                         float *count, float *minv, int *minc, int *sub, int *sup, float *height,
+                         hipStream_t stream);
+
 }  // namespace gt

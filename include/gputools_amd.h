@@ -1,5 +1,6 @@
 /*
- * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library.
+ * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library
+ * and of the hierarchical-clustering stage built on its output.
  *
  * Two groups of entry points:
  *
@@ -66,6 +67,20 @@ enum gt_layout {
     GT_LAYOUT_FULL = 1
 };
 
+/* Clustering methods: same values as hc_method (src/hcluster.h:5-16). */
+enum gt_hc_method {
+    GT_HC_SINGLE = 0,
+    GT_HC_COMPLETE = 1,
+    GT_HC_WPGMA = 2,
+    GT_HC_AVERAGE = 3,
+    GT_HC_MEDIAN = 4,
+    GT_HC_CENTROID = 5,
+    GT_HC_FLEXIBLE_GROUP = 6,
+    GT_HC_FLEXIBLE = 7,
+    GT_HC_WARD = 8,
+    GT_HC_MCQUITTY = 9
+};
+
 enum gt_status {
     GT_OK = 0,
     GT_ERR_INVALID = 1,     /* bad argument: method, size, pointer, p */
@@ -95,13 +110,20 @@ void Rdistances_packed(const float *points, const int *numPoints, const int *dim
                        float *distances, const char **method, const float *p);
 
 /* Replaces Rdistclust (src/rinterface.cpp:172-204, decl src/rinterface.h:50-52).
- * The distance stage runs (device-full layout, p = 2 as at
- * src/rinterface.cpp:182); the hierarchical-clustering stage
- * (src/hcluster.cpp) is not part of this library, so the call then fails
- * with GT_ERR_UNSUPPORTED and leaves merge/order/val untouched. */
+ * Distance stage in the device-full layout with p = 2 (src/rinterface.cpp:182),
+ * then the clustering stage on that matrix (gt_hclust, lambda = beta = 0.5),
+ * then R's hclust merge matrix (column-major (n-1) x 2), order and heights, as
+ * formatClustering builds them (src/rinterface.cpp:241-292).  Method names map
+ * as getClusterEnum does (gt_hclust_method_from_name). */
 void Rdistclust(const char **distmethod, const char **clustmethod,
                 const float *points, const int *numPoints, const int *dim,
                 int *merge, int *order, float *val);
+
+/* Replaces Rhcluster (src/rinterface.cpp:216-239, decl src/rinterface.h:56-57):
+ * clustering of a host numPoints x numPoints distance matrix (what gpuHclust
+ * passes, R/gpuHclust.R:70-77), lambda = beta = 0.5, R-formatted results. */
+void Rhcluster(const float *distMat, const int *numPoints, int *merge, int *order, float *val,
+               const char **method);
 
 /* Replace cuCompile / unloadPackage (src/cudaUtils.cpp:117-165, 167-195),
  * called from R/zzz.R:34-38 and :41-43.  Kernels are precompiled gfx950 code
@@ -166,6 +188,35 @@ int gt_dist_device_full(const float *x_host, int64_t n, int64_t k, int method, f
                         float **d_out, size_t *pitch_bytes);
 
 int gt_free(void *device_ptr);
+
+/* getClusterEnum (src/rinterface.cpp:158-170): exact names "complete",
+ * "wpgma", "average", "median", "centroid", "flexible_group", "flexible",
+ * "ward", "mcquitty"; anything else (including "flexible group") ->
+ * GT_HC_SINGLE. */
+int gt_hclust_method_from_name(const char *name);
+
+/* Bytes of device workspace gt_hclust needs for n points. */
+int gt_hclust_workspace_size(int64_t n, size_t *bytes);
+
+/* Hierarchical clustering of the full, symmetric n x n FP32 device matrix d
+ * (row stride ldd elements), which is destroyed: the replacement of
+ * hclusterPreparedDistances (src/hcluster.cpp:40-201) and its kernels
+ * (inst/cuda/hcluster.cu:4-400).  Writes the n - 1 merges to device arrays:
+ * at step s the clusters represented by sub[s] < sup[s] (0-based point
+ * indices) merge at height[s]; the merged cluster keeps index sup[s].  Ties
+ * and arithmetic follow the reference bit for bit.  n below 2^24. */
+int gt_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta, int *sub,
+              int *sup, float *height, void *workspace, size_t workspace_bytes, void *stream);
+
+/* gt_hclust on a host matrix (the replacement of hcluster,
+ * src/hcluster.cpp:15-38): host in, host sub/sup/height out, synchronous. */
+int gt_hclust_host(const float *d_host, int64_t n, int64_t ldd, int method, float lambda,
+                   float beta, int *sub, int *sup, float *height);
+
+/* R's hclust merge matrix (merge[s] and merge[s + len], 1-based, negative =
+ * singleton) and order (len + 1 entries, 1-based) from a merge sequence of
+ * length len: formatClustering / getPrintOrder (src/rinterface.cpp:241-292). */
+int gt_hclust_format(int64_t len, const int *sub, const int *sup, int *merge, int *order);
 
 /* Backward of the packed distance vector: dX (n x k, row stride lddx) =
  * sum_j G_ij * d(d_ij)/d(x_i) for euclidean (any precision flag is ignored:

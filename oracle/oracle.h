@@ -1,8 +1,9 @@
 /*
- * oracle/oracle.h -- FP64 CPU restatement of gputools' pairwise-distance path
- * (R stats::dist semantics).  TEST INFRASTRUCTURE ONLY: loaded by tests/,
- * __graft_entry__.smoke() and the cpu_baseline leg of bench.py, never by the
- * product library.
+ * oracle/oracle.h -- CPU restatement of gputools' pairwise-distance path
+ * (R stats::dist semantics, FP64) and of its hierarchical-clustering stage
+ * (the reference's FP32 merge loop).  TEST INFRASTRUCTURE ONLY: loaded by
+ * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, never
+ * by the product library.
  */
 #ifndef GPUTOOLS_ORACLE_H
 #define GPUTOOLS_ORACLE_H
@@ -33,6 +34,18 @@ double oracle_compare_packed(const float *x, int64_t n, int64_t k, int64_t ldx,
                              int nthreads);
 int oracle_dist_backward(const float *x, int64_t n, int64_t k, int method, double p,
                          const double *g, double *dx, int nthreads);
+/* Same values as hc_method (src/hcluster.h:5-16). */
+enum {
+    ORACLE_HC_SINGLE = 0, ORACLE_HC_COMPLETE = 1, ORACLE_HC_WPGMA = 2, ORACLE_HC_AVERAGE = 3,
+    ORACLE_HC_MEDIAN = 4, ORACLE_HC_CENTROID = 5, ORACLE_HC_FLEXIBLE_GROUP = 6,
+    ORACLE_HC_FLEXIBLE = 7, ORACLE_HC_WARD = 8, ORACLE_HC_MCQUITTY = 9
+};
+
+/* hclust_oracle.c: d is a full n x n FP32 matrix (row stride ldd), destroyed. */
+int oracle_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                  int *sub, int *sup, float *height);
+int oracle_hclust_format(int len, const int *sub, const int *sup, int *merge, int *order);
+
 double oracle_mds_stress(const float *y, int64_t n, int64_t q, const double *delta,
                          double *dy, int nthreads);
 

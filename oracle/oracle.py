@@ -1,4 +1,4 @@
-"""ctypes wrapper of the FP64 CPU oracle (oracle/dist_oracle.c).
+"""ctypes wrapper of the CPU oracle (oracle/dist_oracle.c, oracle/hclust_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
 the cpu_baseline leg of ``bench.py``.  The product package never imports it.
@@ -6,7 +6,9 @@ the cpu_baseline leg of ``bench.py``.  The product package never imports it.
 The oracle restates R ``stats::dist`` in FP64 for the six methods of gputools'
 ``gpuDist`` (R/gpuHclust.R:1-39, kernels inst/cuda/distance.cu:40-493); see the
 header of dist_oracle.c for the per-method citations and where R and gputools
-differ.
+differ.  It also restates the clustering stage of gpuDistClust / gpuHclust
+(src/hcluster.cpp:40-201, inst/cuda/hcluster.cu, src/rinterface.cpp:241-292)
+in the reference's own FP32 arithmetic; see hclust_oracle.c.
 """
 from __future__ import annotations
 
@@ -21,6 +23,9 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 METHODS = ("euclidean", "maximum", "manhattan", "canberra", "binary", "minkowski")
+# hc_method order (src/hcluster.h:5-16)
+HC_METHODS = ("single", "complete", "wpgma", "average", "median", "centroid", "flexible_group",
+              "flexible", "ward", "mcquitty")
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -61,6 +66,12 @@ def lib():
         L.oracle_mds_stress.argtypes = [_f32p, _i64, _i64, _f64p, ctypes.c_void_p,
                                         ctypes.c_int]
         L.oracle_mds_stress.restype = ctypes.c_double
+        _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.oracle_hclust.argtypes = [_f32p, _i64, _i64, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_float, _i32p, _i32p, _f32p]
+        L.oracle_hclust.restype = ctypes.c_int
+        L.oracle_hclust_format.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p]
+        L.oracle_hclust_format.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -149,3 +160,33 @@ def mds_stress(y, delta, with_grad=True, nthreads=0):
 
 def packed_index(n, i, j) -> int:
     return int(lib().oracle_packed_index(n, i, j))
+
+
+def hclust(d_full, method="complete", lam=0.5, beta=0.5):
+    """The reference's merge loop on a full n x n FP32 distance matrix (a copy
+    is consumed).  Returns (sub, sup, height): at step s clusters represented
+    by sub[s] < sup[s] merge at height[s] (0-based point indices)."""
+    d = np.array(d_full, dtype=np.float32, order="C", copy=True)
+    n = d.shape[0]
+    if d.ndim != 2 or d.shape[1] != n:
+        raise ValueError("d_full must be square")
+    m = HC_METHODS.index(method) if isinstance(method, str) else int(method)
+    sub = np.zeros(max(n - 1, 0), np.int32)
+    sup = np.zeros(max(n - 1, 0), np.int32)
+    h = np.zeros(max(n - 1, 0), np.float32)
+    if lib().oracle_hclust(d, n, n, m, lam, beta, sub, sup, h) != 0:
+        raise ValueError("oracle_hclust rejected its arguments")
+    return sub, sup, h
+
+
+def hclust_format(sub, sup):
+    """R's hclust merge matrix ((n-1) x 2, 1-based, negative = singleton) and
+    order (1-based), as formatClustering / getPrintOrder build them."""
+    sub = np.ascontiguousarray(sub, dtype=np.int32)
+    sup = np.ascontiguousarray(sup, dtype=np.int32)
+    ln = sub.shape[0]
+    merge = np.zeros(2 * ln, np.int32)
+    order = np.zeros(ln + 1, np.int32)
+    if lib().oracle_hclust_format(ln, sub, sup, merge, order) != 0:
+        raise ValueError("oracle_hclust_format rejected its arguments")
+    return merge.reshape(2, ln).T.copy(), order
