@@ -191,6 +191,27 @@ def secondary_paths(torch, gt, wl, steps=3, warmup=1):
     o = torch.empty(1000 * 999 // 2, dtype=torch.float32, device="cuda")
     out["uniform_1000x64_euclidean_ms"] = round(
         _timed(lambda: gt.dist(x, "euclidean", out=o), 20, 3, torch), 4)
+
+    # clustering stage (gpuDistClust's second half) on the 8192 x 256 N(0,1)
+    # euclidean matrix: n - 1 sequential merge rounds on one workgroup, so the
+    # figure of merit is time per round.  The reference's loop rescans the
+    # lower triangle every round: n^3/6 FP32 reads, whose HBM time alone is
+    # given as the reference's floor.
+    n = 8192
+    x = torch.from_numpy(wl.normal(n, 256, seed=2)).cuda()
+    d0 = gt.dist(x, "euclidean", layout="full")
+    hc = {"reference_floor_ms": round(n ** 3 / 6 * 4 / 8e12 * 1e3, 2)}
+    for m in ("single", "complete", "average", "ward"):
+        d = d0.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        gt.hclust(d, m)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        hc[m] = {"ms": round(ms, 2), "us_per_round": round(1e3 * ms / (n - 1), 2)}
+    out["hclust_8192"] = hc
     return out
 
 

@@ -39,6 +39,7 @@ namespace {
 constexpr int HTHREADS = 1024;
 constexpr int HWAVES = HTHREADS / 64;
 constexpr int RESCAN_CAP = 2048;   // rows listed for rescan per round (LDS)
+constexpr int HB = 4;              // columns / rows per thread per batch
 constexpr int NO_COL = -1;         // row without a candidate
 constexpr int PENDING = -2;        // row queued for a rescan
 
@@ -53,6 +54,7 @@ struct HclustArgs {
     int *minc;         // its column, NO_COL or PENDING
     int *sub, *sup;
     float *height;
+    bool vec4;         // rows start 16 B aligned
 };
 
 // Order of equal values in the reference's 32-lane reductions: lane t keeps
@@ -85,19 +87,40 @@ __device__ __forceinline__ void wave_argmin(float &v, int &i)
     }
 }
 
-// argmin of row r over columns 0..r-1 by one wave (all lanes return it)
-__device__ __forceinline__ void scan_row(const float *row, int r, float &bv, int &bi)
+__device__ __forceinline__ void take(float v, int c, float &bv, int &bi)
+{
+    if (v < INFINITY && (bi == NO_COL || beats(v, c, bv, bi))) {
+        bv = v;
+        bi = c;
+    }
+}
+
+// argmin of row r over columns 0..r-1 by one wave (all lanes return it).
+// Rows that start 16 B aligned are read 16 B per lane, four loads in flight.
+__device__ __forceinline__ void scan_row(const float *row, int r, bool vec4, float &bv, int &bi)
 {
     const int lane = threadIdx.x & 63;
     bv = INFINITY;
     bi = NO_COL;
-    for (int c = lane; c < r; c += 64) {
-        const float v = row[c];
-        if (v < INFINITY && (bi == NO_COL || beats(v, c, bv, bi))) {
-            bv = v;
-            bi = c;
+    int c = 0;
+    if (vec4) {
+        const int full = r & ~1023;   // columns covered by whole 4 x 256 blocks
+        for (; c < full; c += 1024) {
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                q[u] = *reinterpret_cast<const float4 *>(row + c + u * 256 + lane * 4);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int cc = c + u * 256 + lane * 4;
+                take(q[u].x, cc, bv, bi);
+                take(q[u].y, cc + 1, bv, bi);
+                take(q[u].z, cc + 2, bv, bi);
+                take(q[u].w, cc + 3, bv, bi);
+            }
         }
     }
+    for (int cc = c + lane; cc < r; cc += 64) take(row[cc], cc, bv, bi);
     wave_argmin(bv, bi);
 }
 
@@ -119,10 +142,11 @@ __device__ __forceinline__ double sep(double x)
 // bot = D[sup][c], top = D[sub][c]; nr = new size of sup, np = size of sub,
 // nq = old size of sup, nk = count[c], val = this round's merge height.
 // Every product is rounded separately, as the reference's unfused code does.
-__device__ __forceinline__ float lw_update(int method, float bot, float top, float nr, float np,
-                                           float nq, float nk, float val, float lambda, float beta)
+template <int METHOD>
+__device__ __forceinline__ float lw_update(float bot, float top, float nr, float np, float nq,
+                                           float nk, float val, float lambda, float beta)
 {
-    switch (method) {
+    switch (METHOD) {
     case HC_SINGLE:
         return fminf(bot, top);
     case HC_COMPLETE:
@@ -170,7 +194,7 @@ __global__ __launch_bounds__(256) void hclust_init_kernel(HclustArgs a)
         }
         float bv;
         int bi;
-        scan_row(row, (int)r, bv, bi);
+        scan_row(row, (int)r, a.vec4, bv, bi);
         if (lane == 0) {
             a.minv[r] = bv;
             a.minc[r] = bi;
@@ -178,6 +202,7 @@ __global__ __launch_bounds__(256) void hclust_init_kernel(HclustArgs a)
     }
 }
 
+template <int METHOD>
 __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
 {
     __shared__ float red_v[HWAVES];
@@ -252,35 +277,67 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
         // ---- rewrite row/column sup; row/column sub become +inf ----------
         float *const dcw = a.d + (int64_t)cw * ldd;
         float *const drw = a.d + (int64_t)rw * ldd;
-        for (int c = tid; c < n; c += HTHREADS) {
-            float bot = INFINITY;
-            if (c != cw && c != rw)
-                bot = lw_update(a.method, drw[c], dcw[c], nr, np, nq, a.count[c], val, a.lambda,
-                                a.beta);
-            dcw[c] = INFINITY;
-            a.d[(int64_t)c * ldd + cw] = INFINITY;
-            drw[c] = bot;
-            a.d[(int64_t)c * ldd + rw] = bot;
+        // batches of HB columns per thread: every load of a batch is issued
+        // before its stores, which could otherwise alias them
+        for (int c0 = tid; c0 < n; c0 += HTHREADS * HB) {
+            float bot[HB], top[HB], nk[HB];
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+                const int c = c0 + u * HTHREADS;
+                if (c < n && c != cw && c != rw) {
+                    bot[u] = drw[c];
+                    top[u] = dcw[c];
+                    nk[u] = a.count[c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+                const int c = c0 + u * HTHREADS;
+                if (c >= n) break;
+                float v = INFINITY;
+                if (c != cw && c != rw)
+                    v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nk[u], val, a.lambda, a.beta);
+                dcw[c] = INFINITY;
+                a.d[(int64_t)c * ldd + cw] = INFINITY;
+                drw[c] = v;
+                a.d[(int64_t)c * ldd + rw] = v;
+            }
         }
         __syncthreads();
 
         // ---- repair the cached row minima --------------------------------
-        for (int r = cw + tid; r < n; r += HTHREADS) {
+        for (int r0 = cw + tid; r0 < n; r0 += HTHREADS * HB) {
+            float cnt[HB], mvs[HB], fs[HB];
+            int mcs[HB];
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+                const int r = r0 + u * HTHREADS;
+                if (r < n) {
+                    cnt[u] = a.count[r];
+                    mvs[u] = a.minv[r];
+                    mcs[u] = a.minc[r];
+                    fs[u] = drw[r];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+            const int r = r0 + u * HTHREADS;
+            if (r >= n) break;
             bool redo = false;
             if (r == cw) {
                 a.minv[r] = INFINITY;
                 a.minc[r] = NO_COL;
                 continue;
             }
-            if (a.count[r] <= 0.0f) continue;
-            const int mc = a.minc[r];
-            const float mv = a.minv[r];
+            if (cnt[u] <= 0.0f) continue;
+            const int mc = mcs[u];
+            const float mv = mvs[u];
             if (r == rw) {
                 redo = true;
             } else if (r < rw) {
                 redo = (mc == cw);
             } else {
-                const float f = drw[r];   // the new D[r][sup]
+                const float f = fs[u];   // the new D[r][sup]
                 if (mc == cw) {
                     redo = !(f < mv);
                     if (!redo) {
@@ -303,6 +360,7 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
                 else
                     s_overflow = 1;
             }
+            }
         }
         __syncthreads();
 
@@ -312,7 +370,7 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             const int r = rescan[q];
             float v;
             int c;
-            scan_row(a.d + (int64_t)r * ldd, r, v, c);
+            scan_row(a.d + (int64_t)r * ldd, r, a.vec4, v, c);
             if (lane == 0) {
                 a.minv[r] = v;
                 a.minc[r] = c;
@@ -324,7 +382,7 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
                 if (a.minc[r] != PENDING) continue;
                 float v;
                 int c;
-                scan_row(a.d + (int64_t)r * ldd, r, v, c);
+                scan_row(a.d + (int64_t)r * ldd, r, a.vec4, v, c);
                 if (lane == 0) {
                     a.minv[r] = v;
                     a.minc[r] = c;
@@ -355,12 +413,31 @@ hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lam
     a.sub = sub;
     a.sup = sup;
     a.height = height;
+    a.vec4 = (ldd % 4 == 0) && ((reinterpret_cast<uintptr_t>(d) & 15) == 0);
     int64_t blocks = (n + 3) / 4;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(hclust_init_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(hclust_merge_kernel, dim3(1), dim3(HTHREADS), 0, stream, a);
+    switch (method) {
+#define GT_HC_CASE(m)                                                                          \
+    case m:                                                                                    \
+        hipLaunchKernelGGL(hclust_merge_kernel<m>, dim3(1), dim3(HTHREADS), 0, stream, a);     \
+        break;
+    GT_HC_CASE(HC_SINGLE)
+    GT_HC_CASE(HC_COMPLETE)
+    GT_HC_CASE(HC_WPGMA)
+    GT_HC_CASE(HC_AVERAGE)
+    GT_HC_CASE(HC_MEDIAN)
+    GT_HC_CASE(HC_CENTROID)
+    GT_HC_CASE(HC_FLEXIBLE_GROUP)
+    GT_HC_CASE(HC_FLEXIBLE)
+    GT_HC_CASE(HC_WARD)
+    GT_HC_CASE(HC_MCQUITTY)
+#undef GT_HC_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
