@@ -1,5 +1,6 @@
-"""gputools_amd -- MI355X-native pairwise distances (gputools' gpuDist path)
-and the clustering stage of gpuDistClust / gpuHclust built on them.
+"""gputools_amd -- MI355X-native pairwise distances (gputools' gpuDist path),
+the clustering stage of gpuDistClust / gpuHclust built on them, and gpuCor's
+all-pairs correlations.
 
 The compute is in libgputools_amd.so (hand-written gfx950 HIP kernels behind a
 C-ABI, include/gputools_amd.h); this package is the host-side mirror of the
@@ -10,8 +11,9 @@ from .dist import (Dist, Rdistances, Rdistances_packed, dist, dist_cross, dist_d
                    free_device, gpuDist, match_method, pmatch, workspace_size)
 from .autograd import dist_backward, mds_stress, pdist  # noqa: F401
 from .hclust import HClust, gpuDistClust, gpuHclust, hclust, hclust_format  # noqa: F401
+from .cor import gpuCor, gpuTtest, pmcc, ttest  # noqa: F401
 
 __all__ = ["Dist", "GputoolsError", "LibraryNotBuilt", "Rdistances", "Rdistances_packed", "dist", "dist_cross",
            "dist_device_full", "free_device", "gpuDist", "lib", "match_method", "pmatch",
            "workspace_size", "dist_backward", "mds_stress", "pdist", "HClust", "gpuDistClust",
-           "gpuHclust", "hclust", "hclust_format"]
+           "gpuHclust", "hclust", "hclust_format", "gpuCor", "gpuTtest", "pmcc", "ttest"]

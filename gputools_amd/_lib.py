@@ -21,6 +21,7 @@ LAYOUT_PACKED, LAYOUT_FULL = 0, 1
 # gt_hc_method (hc_method order, src/hcluster.h:5-16); names as getClusterEnum spells them
 HC_METHOD_NAMES = ("single", "complete", "wpgma", "average", "median", "centroid",
                    "flexible_group", "flexible", "ward", "mcquitty")
+USE_EVERYTHING, USE_PAIRWISE = 0, 1
 GT_OK, GT_ERR_INVALID, GT_ERR_HIP, GT_ERR_ALLOC, GT_ERR_UNSUPPORTED, GT_ERR_WORKSPACE = range(6)
 STATUS_NAMES = {0: "GT_OK", 1: "GT_ERR_INVALID", 2: "GT_ERR_HIP", 3: "GT_ERR_ALLOC",
                 4: "GT_ERR_UNSUPPORTED", 5: "GT_ERR_WORKSPACE"}
@@ -84,6 +85,14 @@ SIGNATURES = {
                           c_p]),
     "gt_hclust_host": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p]),
     "gt_hclust_format": (c_int, [c_i64, c_p, c_p, c_p, c_p]),
+    "rpmcc": (None, [c_ip, c_fp, c_ip, c_fp, c_ip, c_ip, c_fp, c_fp, c_fp]),
+    "rtestT": (None, [c_fp, c_fp, c_ip, c_fp]),
+    "gt_pmcc_workspace_size": (c_int, [c_int, c_int, c_i64, c_i64, c_i64, c_int,
+                                       ctypes.POINTER(c_size)]),
+    "gt_pmcc": (c_int, [c_int, c_int, c_p, c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p,
+                        c_i64, c_p, c_size, c_p]),
+    "gt_pmcc_host": (c_int, [c_int, c_int, c_p, c_i64, c_p, c_i64, c_i64, c_p, c_p, c_p]),
+    "gt_ttest": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
 }
 
 

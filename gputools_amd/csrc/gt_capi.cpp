@@ -382,6 +382,99 @@ int hclust_format(int64_t len, const int *sub, const int *sup, int *merge, int *
     return GT_OK;
 }
 
+// ------------------------------ correlation --------------------------------
+
+int check_pmcc_args(int use, int precision, const float *a, int64_t na, int64_t lda,
+                    const float *b, int64_t nb, int64_t ldb, int64_t dim, const float *pairs,
+                    const float *corr, const float *ts, int64_t ldo)
+{
+    if (use != GT_USE_EVERYTHING && use != GT_USE_PAIRWISE)
+        return fail(GT_ERR_INVALID, "unknown use method %d", use);
+    if (precision < GT_PREC_FP32 || precision > GT_PREC_FP16X3)
+        return fail(GT_ERR_INVALID, "unknown precision %d", precision);
+    if (use == GT_USE_PAIRWISE && precision != GT_PREC_FP32)
+        return fail(GT_ERR_UNSUPPORTED,
+                    "pairwise.complete.obs runs in FP32 (per-pair means are not a matrix product)");
+    if (na < 0 || nb < 0 || dim < 0)
+        return fail(GT_ERR_INVALID, "negative size (na=%lld, nb=%lld, dim=%lld)", (long long)na,
+                    (long long)nb, (long long)dim);
+    // sample counts are carried as FP32 values (the reference's numPairs)
+    if (dim >= (int64_t(1) << 24))
+        return fail(GT_ERR_INVALID, "dim=%lld: sample counts must stay below 2^24", (long long)dim);
+    if (na > (int64_t)std::numeric_limits<int>::max() || nb > (int64_t)std::numeric_limits<int>::max())
+        return fail(GT_ERR_INVALID, "too many variables");
+    if (lda < dim || (b != nullptr && ldb < dim))
+        return fail(GT_ERR_INVALID, "leading dimension below dim");
+    if (ldo < nb) return fail(GT_ERR_INVALID, "ldo=%lld < nb=%lld", (long long)ldo, (long long)nb);
+    if (na > 0 && nb > 0) {
+        if (!pairs || !corr || !ts) return fail(GT_ERR_INVALID, "null output pointer");
+        if (dim > 0 && !a) return fail(GT_ERR_INVALID, "null input pointer");
+    }
+    return GT_OK;
+}
+
+// b == nullptr, or b == a with the same shape: correlations of a with itself
+// (symmetric result, half the tiles).
+bool pmcc_same(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb, int64_t ldb)
+{
+    return b == nullptr || (b == a && nb == na && ldb == lda);
+}
+
+int pmcc_host(int use, int precision, const float *a_host, int64_t na, const float *b_host,
+              int64_t nb, int64_t dim, float *pairs, float *corr, float *ts)
+{
+    int s = check_pmcc_args(use, precision, a_host, na, dim, b_host, nb, dim, dim, pairs, corr,
+                            ts, nb);
+    if (s != GT_OK) return s;
+    if (na == 0 || nb == 0) return GT_OK;
+    // gpuCor(x) passes x twice (R/gpuCor.R:7-9): equal contents are the
+    // symmetric case, as equal pointers are for pmcc (src/correlation.cpp:73)
+    const bool same = b_host == nullptr || b_host == a_host ||
+                      (na == nb && memcmp(a_host, b_host, (size_t)na * (size_t)dim * sizeof(float)) == 0);
+    const size_t out = (size_t)na * (size_t)nb;
+    const size_t ws_bytes = gt::pmcc_workspace_bytes(use, precision, na, nb, same, dim);
+    DeviceBuffer da, db, dout, dws;
+    const size_t abytes = (size_t)na * (size_t)dim * sizeof(float);
+    const size_t bbytes = (size_t)nb * (size_t)dim * sizeof(float);
+    if ((s = da.alloc(abytes, "hipMalloc(variables a)")) != GT_OK) return s;
+    if (!same && (s = db.alloc(bbytes, "hipMalloc(variables b)")) != GT_OK) return s;
+    if ((s = dout.alloc(3 * round_up(out * sizeof(float), 256), "hipMalloc(correlations)")) != GT_OK)
+        return s;
+    if ((s = dws.alloc(ws_bytes, "hipMalloc(workspace)")) != GT_OK) return s;
+    if (abytes) GT_HIP(hipMemcpy(da.ptr, a_host, abytes, hipMemcpyHostToDevice));
+    if (!same && bbytes) GT_HIP(hipMemcpy(db.ptr, b_host, bbytes, hipMemcpyHostToDevice));
+    char *o = static_cast<char *>(dout.ptr);
+    const size_t part = round_up(out * sizeof(float), 256);
+    float *dp = reinterpret_cast<float *>(o), *dc = reinterpret_cast<float *>(o + part),
+          *dt = reinterpret_cast<float *>(o + 2 * part);
+    const float *dbp = same ? static_cast<const float *>(da.ptr) : static_cast<const float *>(db.ptr);
+    GT_HIP(gt::launch_pmcc(use, precision, static_cast<const float *>(da.ptr), na, dim, dbp, nb, dim,
+                           same, dim, dp, dc, dt, nb, dws.ptr, nullptr));
+    GT_HIP(hipMemcpy(pairs, dp, out * sizeof(float), hipMemcpyDeviceToHost));
+    GT_HIP(hipMemcpy(corr, dc, out * sizeof(float), hipMemcpyDeviceToHost));
+    GT_HIP(hipMemcpy(ts, dt, out * sizeof(float), hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+int ttest_host(const float *pairs, const float *corr, int64_t n, float *ts)
+{
+    if (n < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (n == 0) return GT_OK;
+    if (!pairs || !corr || !ts) return fail(GT_ERR_INVALID, "null pointer");
+    DeviceBuffer d;
+    int s;
+    const size_t part = round_up((size_t)n * sizeof(float), 256);
+    if ((s = d.alloc(3 * part, "hipMalloc(t-test)")) != GT_OK) return s;
+    char *o = static_cast<char *>(d.ptr);
+    float *dp = reinterpret_cast<float *>(o), *dc = reinterpret_cast<float *>(o + part),
+          *dt = reinterpret_cast<float *>(o + 2 * part);
+    GT_HIP(hipMemcpy(dp, pairs, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(dc, corr, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    GT_HIP(gt::launch_ttest(dp, dc, dt, n, nullptr));
+    GT_HIP(hipMemcpy(ts, dt, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -572,6 +665,55 @@ int gt_hclust_format(int64_t len, const int *sub, const int *sup, int *merge, in
     return hclust_format(len, sub, sup, merge, order);
 }
 
+int gt_pmcc_workspace_size(int use, int precision, int64_t na, int64_t nb, int64_t dim, int same,
+                           size_t *bytes)
+{
+    if (bytes == nullptr) return fail(GT_ERR_INVALID, "null bytes pointer");
+    if (na < 0 || nb < 0 || dim < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (use != GT_USE_EVERYTHING && use != GT_USE_PAIRWISE)
+        return fail(GT_ERR_INVALID, "unknown use method %d", use);
+    if (precision < GT_PREC_FP32 || precision > GT_PREC_FP16X3)
+        return fail(GT_ERR_INVALID, "unknown precision %d", precision);
+    *bytes = gt::pmcc_workspace_bytes(use, precision, na, same ? na : nb, same != 0, dim);
+    return GT_OK;
+}
+
+int gt_pmcc(int use, int precision, const float *a, int64_t na, int64_t lda, const float *b,
+            int64_t nb, int64_t ldb, int64_t dim, float *pairs, float *corr, float *ts, int64_t ldo,
+            void *workspace, size_t workspace_bytes, void *stream)
+{
+    const bool same = pmcc_same(a, na, lda, b, nb, ldb);
+    if (b == nullptr) {
+        b = a;
+        nb = na;
+        ldb = lda;
+    }
+    int s = check_pmcc_args(use, precision, a, na, lda, b, nb, ldb, dim, pairs, corr, ts, ldo);
+    if (s != GT_OK) return s;
+    const size_t need = gt::pmcc_workspace_bytes(use, precision, na, nb, same, dim);
+    if (need > 0 && (workspace == nullptr || workspace_bytes < need))
+        return fail(GT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes);
+    GT_HIP(gt::launch_pmcc(use, precision, a, na, lda, b, nb, ldb, same, dim, pairs, corr, ts, ldo,
+                           workspace, static_cast<hipStream_t>(stream)));
+    return GT_OK;
+}
+
+int gt_pmcc_host(int use, int precision, const float *a_host, int64_t na, const float *b_host,
+                 int64_t nb, int64_t dim, float *pairs, float *corr, float *ts)
+{
+    if (b_host == nullptr) nb = na;
+    return pmcc_host(use, precision, a_host, na, b_host, nb, dim, pairs, corr, ts);
+}
+
+int gt_ttest(const float *pairs, const float *corr, int64_t n, float *ts, void *stream)
+{
+    if (n < 0) return fail(GT_ERR_INVALID, "negative size");
+    if (n > 0 && (!pairs || !corr || !ts)) return fail(GT_ERR_INVALID, "null pointer");
+    GT_HIP(gt::launch_ttest(pairs, corr, ts, n, static_cast<hipStream_t>(stream)));
+    return GT_OK;
+}
+
 // --------------------------- .C drop-in exports ---------------------------
 
 void Rdistances(const float *points, const int *numPoints, const int *dim, float *distances,
@@ -650,6 +792,34 @@ void Rhcluster(const float *distMat, const int *numPoints, int *merge, int *orde
                            sup.data(), val);
         if (s == GT_OK) s = hclust_format(n - 1, sub.data(), sup.data(), merge, order);
     }
+    raise_in_r(s);
+}
+
+void rpmcc(const int *whichObs, const float *samplesA, const int *numSamplesA,
+           const float *samplesB, const int *numSamplesB, const int *sampleSize, float *numPairs,
+           float *correlations, float *signifs)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!whichObs || !numSamplesA || !numSamplesB || !sampleSize) {
+        s = fail(GT_ERR_INVALID, "rpmcc: null argument");
+    } else if (*whichObs != 0 && *whichObs != 1) {
+        s = fail(GT_ERR_INVALID, "unknown use method");   // rinterface.cpp:47-48
+    } else {
+        // "everything" on the matrix cores (BF16 three-term split: products
+        // within 2^-23 of FP32's), "pairwise.complete.obs" in FP32
+        const int use = *whichObs;
+        const int prec = use == GT_USE_EVERYTHING ? GT_PREC_BF16X3 : GT_PREC_FP32;
+        s = pmcc_host(use, prec, samplesA, *numSamplesA, samplesB, *numSamplesB, *sampleSize,
+                      numPairs, correlations, signifs);
+    }
+    raise_in_r(s);
+}
+
+void rtestT(const float *pairs, const float *coeffs, const int *n, float *ts)
+{
+    g_err[0] = '\0';
+    const int s = n ? ttest_host(pairs, coeffs, *n, ts) : fail(GT_ERR_INVALID, "rtestT: null argument");
     raise_in_r(s);
 }
 

@@ -15,7 +15,8 @@ enum PlainOp {
     OP_MANHATTAN = 2,  // sum |a-b|                          (distance.cu:192-232)
     OP_CANBERRA = 3,   // sum |a-b|/|a+b|, R's 0/0 rule       (distance.cu:273-316)
     OP_MINK3 = 4,      // sum |a-b|^3, then cube root        (distance.cu:453-493, p = 3)
-    OP_MINKP = 5       // sum |a-b|^p, then ^(1/p)           (distance.cu:453-493)
+    OP_MINKP = 5,      // sum |a-b|^p, then ^(1/p)           (distance.cu:453-493)
+    OP_DOT = 6         // sum a*b (Pearson on standardised variables, correlation.cu:252-290)
 };
 
 enum OutLayout {
@@ -87,5 +88,19 @@ enum HcMethod {
 hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
                          float *count, float *minv, int *minc, int *sub, int *sup, float *height,
                          hipStream_t stream);
+
+// Pearson correlation (gpuCor).  use: 0 everything, 1 pairwise.complete.obs.
+// precision (use 0 only): 0 FP32 VALU, 1 BF16 three-term, 2 FP16 two-term
+// split MFMA.  a: na x dim variables (row stride lda), b likewise; same means
+// b is a (symmetric result).  Outputs na x nb, row stride ldo.
+size_t pmcc_workspace_bytes(int use, int precision, int64_t na, int64_t nb, bool same,
+                            int64_t dim);
+hipError_t launch_pmcc(int use, int precision, const float *a, int64_t na, int64_t lda,
+                       const float *b, int64_t nb, int64_t ldb, bool same, int64_t dim,
+                       float *pairs, float *corr, float *ts, int64_t ldo, void *workspace,
+                       hipStream_t stream);
+// gpuTtest: t-scores of n (pairs, corr) without hostSignif's cap.
+hipError_t launch_ttest(const float *pairs, const float *corr, float *ts, int64_t n,
+                        hipStream_t stream);
 
 }  // namespace gt

@@ -12,6 +12,9 @@
 // Summation is two-level: each 16-feature chunk is summed into a partial that
 // is then added into the running total, which bounds the rounding error of a
 // sum of k positive terms by about (16 + k/16) ulp instead of k ulp.
+//
+// The same tile also serves the FP32 form of gpuCor's Pearson correlation
+// (OP_DOT: the dot product of two standardised variables).
 #include "gt_common.h"
 #include "gt_internal.h"
 
@@ -63,6 +66,8 @@ struct Term {
         } else if (OP == OP_MINK3) {
             const float t = fabsf(a - b);
             part = fmaf(t * t, t, part);
+        } else if (OP == OP_DOT) {
+            part = fmaf(a, b, part);
         } else {  // OP_MINKP: |t|^p = 2^(p log2|t|); log2(0) = -inf gives 0
             const float t = fabsf(a - b);
             part += exp2f(p * log2f(t));
@@ -72,7 +77,7 @@ struct Term {
     static __device__ __forceinline__ float finish(float acc, int cnt, float inv_p, float fk)
     {
         if (OP == OP_EUCLID) return sqrtf(acc);
-        if (OP == OP_MAXIMUM || OP == OP_MANHATTAN) return acc;
+        if (OP == OP_MAXIMUM || OP == OP_MANHATTAN || OP == OP_DOT) return acc;
         if (OP == OP_CANBERRA) {
             if (cnt == 0) return __builtin_nanf("");
             return ((float)cnt == fk) ? acc : acc / ((float)cnt / fk);
@@ -249,6 +254,7 @@ hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t 
     case OP_CANBERRA: return launch_op<OP_CANBERRA>(layout, args, total, stream);
     case OP_MINK3: return launch_op<OP_MINK3>(layout, args, total, stream);
     case OP_MINKP: return launch_op<OP_MINKP>(layout, args, total, stream);
+    case OP_DOT: return launch_op<OP_DOT>(layout, args, total, stream);
     default: return hipErrorInvalidValue;
     }
 }

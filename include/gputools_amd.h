@@ -1,6 +1,8 @@
 /*
- * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library
- * and of the hierarchical-clustering stage built on its output.
+ * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library,
+ * of the hierarchical-clustering stage built on its output, and of gpuCor's
+ * Pearson correlation (the same all-pairs contraction on standardised
+ * variables).
  *
  * Two groups of entry points:
  *
@@ -81,6 +83,13 @@ enum gt_hc_method {
     GT_HC_MCQUITTY = 9
 };
 
+/* Sample handling of the Pearson correlation: same values as whichObs
+ * (src/rinterface.cpp:31-50). */
+enum gt_use {
+    GT_USE_EVERYTHING = 0,   /* every sample; NaN propagates */
+    GT_USE_PAIRWISE = 1      /* per pair, the samples where neither value is NaN */
+};
+
 enum gt_status {
     GT_OK = 0,
     GT_ERR_INVALID = 1,     /* bad argument: method, size, pointer, p */
@@ -124,6 +133,24 @@ void Rdistclust(const char **distmethod, const char **clustmethod,
  * passes, R/gpuHclust.R:70-77), lambda = beta = 0.5, R-formatted results. */
 void Rhcluster(const float *distMat, const int *numPoints, int *merge, int *order, float *val,
                const char **method);
+
+/* Replaces rpmcc (src/rinterface.cpp:33-55, body pmcc at
+ * src/correlation.cpp:66-160), what gpuCor(method = "pearson") calls
+ * (R/gpuCor.R:33-37).  Variable x of A is the sampleSize floats at
+ * samplesA + x * sampleSize (a column of R's matrix).  Writes
+ * numSamplesA x numSamplesB row-major numPairs, correlations and t-scores
+ * (hostSignif, src/correlation.cpp:51-64).  whichObs 0 = "everything"
+ * (split-precision MFMA, FP32-equivalent), 1 = "pairwise.complete.obs"
+ * (FP32).  Identical A and B contents are computed as one symmetric matrix. */
+void rpmcc(const int *whichObs, const float *samplesA, const int *numSamplesA,
+           const float *samplesB, const int *numSamplesB, const int *sampleSize, float *numPairs,
+           float *correlations, float *signifs);
+
+/* Replaces rtestT (src/rinterface.cpp:87-91, body testSignif at
+ * src/correlation.cpp:17-49), what gpuTtest calls (R/gpuCor.R:68-77):
+ * t = coeff * sqrt((pairs - 2) / (1 - coeff^2)) for n entries, on the GPU,
+ * without hostSignif's cap. */
+void rtestT(const float *pairs, const float *coeffs, const int *n, float *ts);
 
 /* Replace cuCompile / unloadPackage (src/cudaUtils.cpp:117-165, 167-195),
  * called from R/zzz.R:34-38 and :41-43.  Kernels are precompiled gfx950 code
@@ -255,6 +282,27 @@ int gt_dist_batched_schedule(const int64_t *sizes_host, const int64_t *dims_host
 int gt_dist_batched(const float *x, const int64_t *x_offsets, const int64_t *sizes,
                     const int64_t *dims, int64_t nsets, float *out, const int64_t *out_offsets,
                     int method, int tile, const int64_t *schedule, int64_t count, void *stream);
+
+/* Pearson correlation of every variable of A (na x dim, row stride lda >= dim)
+ * with every variable of B (nb x dim, stride ldb), the device form of pmcc
+ * (src/correlation.cpp:66-160).  Outputs na x nb, row stride ldo >= nb, pair
+ * (x, y) at x * ldo + y: the sample count, the correlation and hostSignif's
+ * t-score.  b == NULL (or b == a with the same shape) correlates A with
+ * itself: the result is exactly symmetric and half the tiles are computed.
+ * use GT_USE_EVERYTHING: precision FP32 (VALU), BF16X3 or FP16X3 (MFMA on
+ * standardised variables); GT_USE_PAIRWISE: FP32 only.  The workspace
+ * (gt_pmcc_workspace_size, same != 0 for the self case) holds the
+ * standardised variables; none is needed for GT_USE_PAIRWISE. */
+int gt_pmcc_workspace_size(int use, int precision, int64_t na, int64_t nb, int64_t dim, int same,
+                           size_t *bytes);
+int gt_pmcc(int use, int precision, const float *a, int64_t na, int64_t lda, const float *b,
+            int64_t nb, int64_t ldb, int64_t dim, float *pairs, float *corr, float *ts, int64_t ldo,
+            void *workspace, size_t workspace_bytes, void *stream);
+/* Host-memory form (contiguous rows; b_host == NULL for the self case). */
+int gt_pmcc_host(int use, int precision, const float *a_host, int64_t na, const float *b_host,
+                 int64_t nb, int64_t dim, float *pairs, float *corr, float *ts);
+/* gpuSignif (inst/cuda/correlation.cu:292-310) on device arrays of n entries. */
+int gt_ttest(const float *pairs, const float *corr, int64_t n, float *ts, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,9 @@
 /*
  * oracle/oracle.h -- CPU restatement of gputools' pairwise-distance path
- * (R stats::dist semantics, FP64) and of its hierarchical-clustering stage
- * (the reference's FP32 merge loop).  TEST INFRASTRUCTURE ONLY: loaded by
- * tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, never
- * by the product library.
+ * (R stats::dist semantics, FP64), of its hierarchical-clustering stage
+ * (the reference's FP32 merge loop) and of gpuCor's correlations.  TEST
+ * INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, never by the product library.
  */
 #ifndef GPUTOOLS_ORACLE_H
 #define GPUTOOLS_ORACLE_H
@@ -45,6 +45,11 @@ enum {
 int oracle_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
                   int *sub, int *sup, float *height);
 int oracle_hclust_format(int len, const int *sub, const int *sup, int *merge, int *order);
+
+/* cor_oracle.c: Pearson correlation (pmcc) in FP64, and the FP32 t-scores. */
+int oracle_pmcc(int use, const float *a, int64_t na, const float *b, int64_t nb, int64_t dim,
+                double *pairs, double *corr, int nthreads);
+void oracle_signif(const float *pairs, const float *corr, int64_t n, int cap, float *ts);
 
 double oracle_mds_stress(const float *y, int64_t n, int64_t q, const double *delta,
                          double *dy, int nthreads);

@@ -8,7 +8,9 @@ The oracle restates R ``stats::dist`` in FP64 for the six methods of gputools'
 header of dist_oracle.c for the per-method citations and where R and gputools
 differ.  It also restates the clustering stage of gpuDistClust / gpuHclust
 (src/hcluster.cpp:40-201, inst/cuda/hcluster.cu, src/rinterface.cpp:241-292)
-in the reference's own FP32 arithmetic; see hclust_oracle.c.
+in the reference's own FP32 arithmetic; see hclust_oracle.c.  And it restates
+gpuCor's Pearson correlation (src/correlation.cpp:51-160,
+inst/cuda/correlation.cu) in FP64 with the t-scores in FP32; see cor_oracle.c.
 """
 from __future__ import annotations
 
@@ -72,6 +74,11 @@ def lib():
         L.oracle_hclust.restype = ctypes.c_int
         L.oracle_hclust_format.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p]
         L.oracle_hclust_format.restype = ctypes.c_int
+        L.oracle_pmcc.argtypes = [ctypes.c_int, _f32p, _i64, _f32p, _i64, _i64, _f64p, _f64p,
+                                  ctypes.c_int]
+        L.oracle_pmcc.restype = ctypes.c_int
+        L.oracle_signif.argtypes = [_f32p, _f32p, _i64, ctypes.c_int, _f32p]
+        L.oracle_signif.restype = None
         _lib = L
     return _lib
 
@@ -190,3 +197,37 @@ def hclust_format(sub, sup):
     if lib().oracle_hclust_format(ln, sub, sup, merge, order) != 0:
         raise ValueError("oracle_hclust_format rejected its arguments")
     return merge.reshape(2, ln).T.copy(), order
+
+
+USES = ("everything", "pairwise.complete.obs")
+
+
+def pmcc(a, b=None, use="everything", nthreads=0):
+    """Pearson correlation of every variable (row) of ``a`` with every row of
+    ``b`` (default ``a``), as gputools' pmcc computes it, in FP64.  ``a`` is
+    na x dim (row x = variable x: a column of R's matrix).  Returns
+    (pairs, corr), both na x nb FP64."""
+    a = _f32(a)
+    b = a if b is None else _f32(b)
+    if a.ndim != 2 or b.ndim != 2 or a.shape[1] != b.shape[1]:
+        raise ValueError("a and b must be 2-D with the same number of samples")
+    u = USES.index(use) if isinstance(use, str) else int(use)
+    na, dim = a.shape
+    nb = b.shape[0]
+    pairs = np.empty(na * nb, dtype=np.float64)
+    corr = np.empty(na * nb, dtype=np.float64)
+    if lib().oracle_pmcc(u, a, na, b, nb, dim, pairs, corr, nthreads) != 0:
+        raise ValueError("oracle_pmcc rejected its arguments")
+    return pairs.reshape(na, nb), corr.reshape(na, nb)
+
+
+def signif(pairs, corr, cap=True) -> np.ndarray:
+    """FP32 t-scores: hostSignif (cap, what gpuCor returns) or gpuSignif (no
+    cap, what gpuTtest returns)."""
+    p = _f32(pairs).ravel()
+    c = _f32(corr).ravel()
+    if p.shape != c.shape:
+        raise ValueError("pairs and corr differ in size")
+    ts = np.empty_like(c)
+    lib().oracle_signif(p, c, c.shape[0], 1 if cap else 0, ts)
+    return ts.reshape(np.shape(corr))
