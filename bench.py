@@ -49,7 +49,13 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-paths", action="store_true",
-                    help="skip the secondary paths (other methods, batched, backward, MDS)")
+                    help="skip the secondary paths (other methods, batched, backward, MDS, ...)")
+    ap.add_argument("--paths", default=None,
+                    help="comma list of secondary path groups to time (default: all of "
+                         "methods,batched,backward,small,hclust,pmcc)")
+    ap.add_argument("--paths-only", action="store_true",
+                    help="time only the selected secondary paths (no headline line; for "
+                         "profiling them)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (reference kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -117,9 +123,7 @@ def _valu_roof(flops, ms):
             "frac": round(tf / 157.3, 4)}
 
 
-def secondary_paths(torch, gt, wl, steps=3, warmup=1):
-    """Timed secondary paths, each with its algorithmic work and roofline."""
-    out = {}
+def path_methods(torch, gt, wl, steps, warmup, out):
     # all six methods, 8192 x 256 (FP32 VALU kernels; binary on bit rows)
     n, k = 8192, 256
     pairs = n * (n - 1) // 2
@@ -141,6 +145,8 @@ def secondary_paths(torch, gt, wl, steps=3, warmup=1):
         meth[m] = rec
     out["methods_8192x256_fp32"] = meth
 
+
+def path_batched(torch, gt, wl, steps, warmup, out):
     # batched: 4096 sets, log-uniform sizes in [32, 2048], d in {16, 32, 64, 128}
     from gputools_amd.autotune import tune_batched
     from gputools_amd.batched import Batch
@@ -164,6 +170,8 @@ def secondary_paths(torch, gt, wl, steps=3, warmup=1):
                   "roofline": _valu_roof(flops, ms_graph)}
     out["batched_4096_sets"] = bat
 
+
+def path_backward(torch, gt, wl, steps, warmup, out):
     # backward and MDS stress: delta from 16384 x 128 points
     n, k = 16384, 128
     pairs = n * (n - 1) // 2
@@ -186,12 +194,16 @@ def secondary_paths(torch, gt, wl, steps=3, warmup=1):
                            "roofline": _valu_roof(2.0 * pairs * 5 * q, ms)}
     out["mds_stress_16384"] = mds
 
+
+def path_small(torch, gt, wl, steps, warmup, out):
     # the reference's small case: 1000 x 64 uniform, euclidean, latency per call
     x = torch.from_numpy(wl.uniform(1000, 64, seed=1)).cuda()
     o = torch.empty(1000 * 999 // 2, dtype=torch.float32, device="cuda")
     out["uniform_1000x64_euclidean_ms"] = round(
         _timed(lambda: gt.dist(x, "euclidean", out=o), 20, 3, torch), 4)
 
+
+def path_hclust(torch, gt, wl, steps, warmup, out):
     # clustering stage (gpuDistClust's second half) on the 8192 x 256 N(0,1)
     # euclidean matrix: n - 1 sequential merge rounds on one workgroup, so the
     # figure of merit is time per round.  The reference's loop rescans the
@@ -212,6 +224,50 @@ def secondary_paths(torch, gt, wl, steps=3, warmup=1):
         ms = e0.elapsed_time(e1)
         hc[m] = {"ms": round(ms, 2), "us_per_round": round(1e3 * ms / (n - 1), 2)}
     out["hclust_8192"] = hc
+
+
+def path_pmcc(torch, gt, wl, steps, warmup, out):
+    # gpuCor(x), Pearson: 8192 variables x 1024 samples N(0,1), the symmetric
+    # self case (half the tiles computed, every value stored twice).  Each
+    # pair gets pairs, corr and t-score: 12 B of output, which bounds the
+    # "everything" path; its algorithmic work is one 1024-long dot product
+    # per unique pair.  pairwise.complete.obs (10% NaN) is per-pair VALU
+    # work: 3 accumulations (count, two sums) and then 2 differences and 3
+    # fused products per sample, 11 flops.
+    nv, size = 8192, 1024
+    a = torch.from_numpy(wl.normal(nv, size, seed=6)).cuda()
+    outs = tuple(torch.empty((nv, nv), dtype=torch.float32, device="cuda") for _ in range(3))
+    wbytes = 12.0 * nv * nv
+    cor = {}
+    for prec in ("bf16x3", "fp16x3", "fp32"):
+        ms = _timed(lambda: gt.pmcc(a, precision=prec, out=outs), steps, warmup, torch)
+        gbs = wbytes / (ms * 1e-3) / 1e9
+        cor["everything_" + prec] = {
+            "ms": round(ms, 4), "pairs_per_s": round(nv * nv / (ms * 1e-3), 1),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0,
+                         "unit": "GB/s", "frac": round(gbs / 8000.0, 4)}}
+    rng = np.random.default_rng(66)
+    an = a.clone()
+    an[torch.from_numpy(rng.random((nv, size)) < 0.1).cuda()] = float("nan")
+    ms = _timed(lambda: gt.pmcc(an, use="pairwise.complete.obs", precision="fp32", out=outs),
+                steps, warmup, torch)
+    cor["pairwise_complete_fp32"] = {
+        "ms": round(ms, 4), "pairs_per_s": round(nv * nv / (ms * 1e-3), 1),
+        "roofline": _valu_roof(11.0 * size * nv * (nv + 1) / 2, ms)}
+    out["pmcc_self_8192x1024"] = cor
+
+
+PATHS = {"methods": path_methods, "batched": path_batched, "backward": path_backward,
+         "small": path_small, "hclust": path_hclust, "pmcc": path_pmcc}
+
+
+def secondary_paths(torch, gt, wl, select=None, steps=3, warmup=1):
+    """Timed secondary paths, each with its algorithmic work and roofline."""
+    out = {}
+    for name, fn in PATHS.items():
+        if select is None or name in select:
+            print("bench: path %s" % name, file=sys.stderr, flush=True)
+            fn(torch, gt, wl, steps, warmup, out)
     return out
 
 
@@ -228,6 +284,14 @@ def main():
     from gputools_amd import workloads as wl
 
     lib = gt.lib()
+    select = None if args.paths is None else set(p for p in args.paths.split(",") if p)
+    if select is not None and not select <= set(PATHS):
+        print("bench.py: unknown path group(s) %s" % sorted(select - set(PATHS)), file=sys.stderr)
+        sys.exit(2)
+    if args.paths_only:
+        print(json.dumps({"paths": secondary_paths(torch, gt, wl, select, args.steps,
+                                                    args.warmup)}))
+        return
     if args.calibrate_copy:
         a = torch.ones(512 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
@@ -358,7 +422,7 @@ def main():
 
     if not args.no_paths:
         print("bench: secondary paths", file=sys.stderr, flush=True)
-        line["paths"] = secondary_paths(torch, gt, wl)
+        line["paths"] = secondary_paths(torch, gt, wl, select)
 
     print(json.dumps(line), flush=True)
 

@@ -11,9 +11,9 @@ from .dist import (Dist, Rdistances, Rdistances_packed, dist, dist_cross, dist_d
                    free_device, gpuDist, match_method, pmatch, workspace_size)
 from .autograd import dist_backward, mds_stress, pdist  # noqa: F401
 from .hclust import HClust, gpuDistClust, gpuHclust, hclust, hclust_format  # noqa: F401
-from .cor import gpuCor, gpuTtest, pmcc, ttest  # noqa: F401
+from .cor import gpuCor, gpuTtest, kendall, pmcc, ttest  # noqa: F401
 
 __all__ = ["Dist", "GputoolsError", "LibraryNotBuilt", "Rdistances", "Rdistances_packed", "dist", "dist_cross",
            "dist_device_full", "free_device", "gpuDist", "lib", "match_method", "pmatch",
            "workspace_size", "dist_backward", "mds_stress", "pdist", "HClust", "gpuDistClust",
-           "gpuHclust", "hclust", "hclust_format", "gpuCor", "gpuTtest", "pmcc", "ttest"]
+           "gpuHclust", "hclust", "hclust_format", "gpuCor", "gpuTtest", "kendall", "pmcc", "ttest"]

@@ -93,6 +93,11 @@ SIGNATURES = {
                         c_i64, c_p, c_size, c_p]),
     "gt_pmcc_host": (c_int, [c_int, c_int, c_p, c_i64, c_p, c_i64, c_i64, c_p, c_p, c_p]),
     "gt_ttest": (c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "RgpuKendall": (None, [c_fp, c_ip, c_fp, c_ip, c_ip, ctypes.POINTER(ctypes.c_double)]),
+    "gt_kendall_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_size)]),
+    "gt_kendall": (c_int, [c_p, c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_p, c_i64, c_p, c_size,
+                           c_p]),
+    "gt_kendall_host": (c_int, [c_p, c_i64, c_p, c_i64, c_i64, c_p]),
 }
 
 

@@ -12,7 +12,7 @@ unknown or ambiguous name yields -1, so every failed match stops with
 reference, y must have as many rows as x (the reference reads past the end
 of y otherwise).
 
-The device API (:func:`pmcc`, :func:`ttest`) works on torch tensors that
+The device API (:func:`pmcc`, :func:`kendall`, :func:`ttest`) works on torch tensors that
 already live in HBM (variables as rows) and runs on torch's current stream.
 """
 from __future__ import annotations
@@ -89,10 +89,22 @@ def gpuCor(x, y=None, use: str = "everything", method: str = "pearson") -> dict:
         return {"coefficients": corr[:n].reshape(shape).astype(np.float64),
                 "ts": ts[:n].reshape(shape).astype(np.float64),
                 "pairs": pairs[:n].reshape(shape).astype(np.float64)}
+    # kendall (R/gpuCor.R:46-59)
     if use != "everything":
         warnings.warn("NA handling for Kendall's is not yet supported. Defaulting to using "
                       "everything. Sorry for any inconvenience.")
-    raise _lib.GputoolsError(_lib.GT_ERR_UNSUPPORTED, "kendall is not built yet")
+    result = np.zeros(max(n, 1), np.float64)
+    fp = ctypes.POINTER(ctypes.c_float)
+    cnx, cny, csz = ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(size)
+    lib().RgpuKendall(a.ctypes.data_as(fp), ctypes.byref(cnx), b.ctypes.data_as(fp),
+                      ctypes.byref(cny), ctypes.byref(csz),
+                      result.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    err = _lib.last_error()
+    if err:
+        raise _lib.GputoolsError(_lib.GT_ERR_HIP, err)
+    # matrix(a$result, nx, ny): column-major, entry (x, y) at y * nx + x
+    return {"coefficients": result[:n].reshape(ny, nx).T.copy(),
+            "pairs": np.full((nx, ny), float(size))}
 
 
 def gpuTtest(goodPairs, coeffs) -> np.ndarray:
@@ -159,6 +171,38 @@ def pmcc(a, b=None, use="everything", precision: str = "bf16x3", stream=None, ou
                         corr.data_ptr(), ts.data_ptr(), pairs.stride(0), ws.data_ptr(),
                         nbytes.value, s.cuda_stream or None))
     return pairs, corr, ts
+
+
+def kendall(a, b=None, stream=None, out=None):
+    """Kendall's tau (the reference's definition: sample pairs tied in both
+    variables count as concordant) of the rows of ``a`` (na x n float32 CUDA
+    tensor, one variable per row) with the rows of ``b`` (default ``a``).
+    Returns an nb x na float64 tensor, entry (y, x) = tau(a[x], b[y]) (the
+    reference's layout), or fills ``out``."""
+    import torch
+    for t, name in ((a, "a"), (b, "b")):
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
+            raise TypeError("%s must be a float32 torch tensor on the GPU" % name)
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise ValueError("%s must be 2-D with unit column stride" % name)
+    if b is not None and b.shape[1] != a.shape[1]:
+        raise ValueError("a and b must have the same number of samples")
+    na, n = a.shape
+    nb = na if b is None else b.shape[0]
+    if out is None:
+        out = torch.empty((nb, na), dtype=torch.float64, device=a.device)
+    if out.shape != (nb, na) or out.dtype != torch.float64 or out.stride(1) != 1:
+        raise ValueError("out must be an nb x na float64 tensor with unit column stride")
+    nbytes = ctypes.c_size_t()
+    check(lib().gt_kendall_workspace_size(na, nb, n, 1 if b is None else 0, ctypes.byref(nbytes)))
+    ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=a.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib().gt_kendall(a.data_ptr(), na, a.stride(0), None if b is None else b.data_ptr(), nb,
+                           a.stride(0) if b is None else b.stride(0), n, out.data_ptr(),
+                           out.stride(0), ws.data_ptr(), nbytes.value, s.cuda_stream or None))
+    return out
 
 
 def ttest(pairs, corr, stream=None):

@@ -475,6 +475,53 @@ int ttest_host(const float *pairs, const float *corr, int64_t n, float *ts)
     return GT_OK;
 }
 
+int check_kendall_args(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
+                       int64_t ldb, int64_t n, const double *tau, int64_t ldt)
+{
+    if (na < 0 || nb < 0 || n < 0)
+        return fail(GT_ERR_INVALID, "negative size (na=%lld, nb=%lld, n=%lld)", (long long)na,
+                    (long long)nb, (long long)n);
+    // sample-pair words are addressed with 64-bit offsets; 2^22 samples give
+    // 2^37 sample pairs per variable
+    if (n > (int64_t(1) << 22))
+        return fail(GT_ERR_INVALID, "n=%lld samples exceeds the supported 2^22", (long long)n);
+    if (lda < n || (b != nullptr && ldb < n)) return fail(GT_ERR_INVALID, "leading dimension below n");
+    if (ldt < na) return fail(GT_ERR_INVALID, "ldt=%lld < na=%lld", (long long)ldt, (long long)na);
+    if (na > 0 && nb > 0) {
+        if (!tau) return fail(GT_ERR_INVALID, "null output pointer");
+        if (n > 0 && !a) return fail(GT_ERR_INVALID, "null input pointer");
+    }
+    return GT_OK;
+}
+
+int kendall_host(const float *a_host, int64_t na, const float *b_host, int64_t nb, int64_t n,
+                 double *tau)
+{
+    int s = check_kendall_args(a_host, na, n, b_host, nb, n, n, tau, na);
+    if (s != GT_OK) return s;
+    if (na == 0 || nb == 0) return GT_OK;
+    // gpuCor passes y = x as a second copy (R/gpuCor.R:7-9, 52-55): equal
+    // contents are the symmetric case
+    const bool same = b_host == nullptr || b_host == a_host ||
+                      (na == nb && memcmp(a_host, b_host, (size_t)na * (size_t)n * sizeof(float)) == 0);
+    const size_t abytes = (size_t)na * (size_t)n * sizeof(float);
+    const size_t bbytes = (size_t)nb * (size_t)n * sizeof(float);
+    const size_t out = (size_t)na * (size_t)nb * sizeof(double);
+    DeviceBuffer da, db, dout, dws;
+    if ((s = da.alloc(abytes, "hipMalloc(variables a)")) != GT_OK) return s;
+    if (!same && (s = db.alloc(bbytes, "hipMalloc(variables b)")) != GT_OK) return s;
+    if ((s = dout.alloc(out, "hipMalloc(tau)")) != GT_OK) return s;
+    if ((s = dws.alloc(gt::kendall_workspace_bytes(na, nb, same, n), "hipMalloc(workspace)")) != GT_OK)
+        return s;
+    if (abytes) GT_HIP(hipMemcpy(da.ptr, a_host, abytes, hipMemcpyHostToDevice));
+    if (!same && bbytes) GT_HIP(hipMemcpy(db.ptr, b_host, bbytes, hipMemcpyHostToDevice));
+    const float *dbp = same ? static_cast<const float *>(da.ptr) : static_cast<const float *>(db.ptr);
+    GT_HIP(gt::launch_kendall(static_cast<const float *>(da.ptr), na, n, dbp, nb, n, same, n,
+                              static_cast<double *>(dout.ptr), na, dws.ptr, nullptr));
+    GT_HIP(hipMemcpy(tau, dout.ptr, out, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -714,6 +761,42 @@ int gt_ttest(const float *pairs, const float *corr, int64_t n, float *ts, void *
     return GT_OK;
 }
 
+int gt_kendall_workspace_size(int64_t na, int64_t nb, int64_t n, int same, size_t *bytes)
+{
+    if (bytes == nullptr) return fail(GT_ERR_INVALID, "null bytes pointer");
+    if (na < 0 || nb < 0 || n < 0) return fail(GT_ERR_INVALID, "negative size");
+    *bytes = gt::kendall_workspace_bytes(na, same ? na : nb, same != 0, n);
+    return GT_OK;
+}
+
+int gt_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb, int64_t ldb,
+               int64_t n, double *tau, int64_t ldt, void *workspace, size_t workspace_bytes,
+               void *stream)
+{
+    const bool same = b == nullptr || (b == a && nb == na && ldb == lda);
+    if (b == nullptr) {
+        b = a;
+        nb = na;
+        ldb = lda;
+    }
+    int s = check_kendall_args(a, na, lda, b, nb, ldb, n, tau, ldt);
+    if (s != GT_OK) return s;
+    const size_t need = gt::kendall_workspace_bytes(na, nb, same, n);
+    if (need > 0 && (workspace == nullptr || workspace_bytes < need))
+        return fail(GT_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes);
+    GT_HIP(gt::launch_kendall(a, na, lda, b, nb, ldb, same, n, tau, ldt, workspace,
+                              static_cast<hipStream_t>(stream)));
+    return GT_OK;
+}
+
+int gt_kendall_host(const float *a_host, int64_t na, const float *b_host, int64_t nb, int64_t n,
+                    double *tau)
+{
+    if (b_host == nullptr) nb = na;
+    return kendall_host(a_host, na, b_host, nb, n, tau);
+}
+
 // --------------------------- .C drop-in exports ---------------------------
 
 void Rdistances(const float *points, const int *numPoints, const int *dim, float *distances,
@@ -820,6 +903,18 @@ void rtestT(const float *pairs, const float *coeffs, const int *n, float *ts)
 {
     g_err[0] = '\0';
     const int s = n ? ttest_host(pairs, coeffs, *n, ts) : fail(GT_ERR_INVALID, "rtestT: null argument");
+    raise_in_r(s);
+}
+
+void RgpuKendall(const float *X, const int *nx, const float *Y, const int *ny,
+                 const int *sampleSize, double *answers)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!nx || !ny || !sampleSize)
+        s = fail(GT_ERR_INVALID, "RgpuKendall: null argument");
+    else
+        s = kendall_host(X, *nx, Y, *ny, *sampleSize, answers);
     raise_in_r(s);
 }
 

@@ -103,4 +103,12 @@ hipError_t launch_pmcc(int use, int precision, const float *a, int64_t na, int64
 hipError_t launch_ttest(const float *pairs, const float *corr, float *ts, int64_t n,
                         hipStream_t stream);
 
+// Kendall's tau (gpuCor method = "kendall").  a: na x n variables (row
+// stride lda), b likewise; same means b is a.  tau: nb x na FP64, pair (x, y)
+// at y * ldt + x (the reference's layout).  Workspace: bit planes and counts.
+size_t kendall_workspace_bytes(int64_t na, int64_t nb, bool same, int64_t n);
+hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
+                          int64_t ldb, bool same, int64_t n, double *tau, int64_t ldt,
+                          void *workspace, hipStream_t stream);
+
 }  // namespace gt

@@ -1,0 +1,338 @@
+// gt_kendall.hip -- Kendall's tau of gpuCor(method = "kendall") on gfx950.
+//
+// Replaces masterKendall (src/kendall.cpp:13-55) and gpuKendall
+// (inst/cuda/kendall.cu:4-47).  For variables x (of A) and y (of B) over n
+// samples, gputools counts the sample pairs i < j that are concordant, where
+//   (x_j > x_i and y_j > y_i) or (x_j < x_i and y_j < y_i)
+//   or (x_j == x_i and y_j == y_i)                        (kendall.cu:18-22)
+// (any comparison with NaN is false), and returns
+//   (concordant - discordant) / (n (n - 1) / 2),  discordant = that - concordant
+// with the final steps in FP32 and the quotient in FP64 (kendall.cu:40-45).
+// It runs one 256-thread block per pair of variables, each re-comparing the
+// raw values of both variables n^2/2 times.
+//
+// Here the comparisons are made once per variable.  A planes pass encodes
+// each sample pair (i, j), i < j, of a variable as three bits: P (x_j > x_i),
+// N (x_j < x_i) and E (x_j == x_i); a NaN on either side clears all three.
+// A wave compares x_i with 64 consecutive x_j at once and __ballot turns the
+// comparisons into bit words.  The sample pairs of row i take
+// ceil((n - 1 - i) / 32) words (the tail bits are zero), so
+//   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
+//                      + popc(Ex & Ey)
+// exactly, 6 VALU operations per 32 sample pairs per pair of variables (AND
+// and v_bcnt_u32_b32, which adds the count to an accumulator).  The count
+// kernel tiles 128 x 128 pairs of variables over 256 lanes (8 x 8 each),
+// stages word chunks of both sides through LDS and splits the word range over
+// workgroups, adding its counts with 64-bit atomics.  A last pass applies the
+// reference's FP32 / FP64 formula to the exact count.
+//
+// The reference accumulates the count in FP32 (kendall.cu:14, 23, 28-38):
+// below 2^24 concordant pairs (always when n <= 5793) that sum is exact and
+// the results are identical; above it the reference's sum can round, and this
+// build keeps the exact count.
+#include "gt_common.h"
+#include "gt_internal.h"
+
+namespace gt {
+namespace {
+
+// Words of the rows with m = 1 .. M sample pairs: sum_{t=1}^{M} ceil(t / 32).
+__host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
+{
+    const int64_t q = m / 32, r = m % 32;
+    return 16 * q * (q + 1) + r * (q + 1);
+}
+
+// ---------------------------------------------------------------------------
+// Planes pass: one wave per (variable, sample i).
+// ---------------------------------------------------------------------------
+// Plane p (0 = P, 1 = N, 2 = E) of variable v, word w at
+// planes[(p * nvar + v) * wpad + w]; row i starts at word
+// words_upto(n - 1) - words_upto(n - 1 - i).  Words past the last row are
+// zero (cleared before this pass).
+__global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__restrict__ x,
+                                                             int64_t nvar, int64_t n, int64_t ldx,
+                                                             uint32_t *__restrict__ planes,
+                                                             int64_t wpad)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t rows = nvar * n, wtot = words_upto(n - 1);
+    const float qnan = __builtin_nanf("");
+    for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < rows; r += (int64_t)gridDim.x * 4) {
+        const int64_t v = r / n, i = r % n;
+        const int64_t m = n - 1 - i;
+        if (m == 0) continue;
+        const float *xv = x + v * ldx;
+        const float xi = xv[i];
+        const int64_t base = wtot - words_upto(m);
+        const int64_t wr = (m + 31) / 32;
+        // lane l < 6 stores word half (l & 1) of plane (l >> 1)
+        const int plane = lane >> 1, half = lane & 1;
+        uint32_t *dst = planes + ((int64_t)(plane < 3 ? plane : 0) * nvar + v) * wpad + base + half;
+        for (int64_t t = 0; t * 64 < m; ++t) {
+            const int64_t j = i + 1 + t * 64 + lane;
+            const float xj = j < n ? xv[j] : qnan;
+            const unsigned long long gt = __ballot(xj > xi);
+            const unsigned long long lt = __ballot(xj < xi);
+            const unsigned long long eq = __ballot(xj == xi);
+            const unsigned long long mine = plane == 0 ? gt : (plane == 1 ? lt : eq);
+            if (lane < 6 && 2 * t + half < wr) dst[2 * t] = (uint32_t)(mine >> (32 * half));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Count pass: popcount tiles.
+// ---------------------------------------------------------------------------
+constexpr int KBM = 128;                 // variables per tile side
+constexpr int KKW = 8;                   // words per LDS chunk (256 sample pairs)
+constexpr int KTHREADS = 256;            // 16 x 16 lanes, 8 x 8 pairs each
+constexpr int KUNITS = 2 * 3 * KBM * 2 / KTHREADS;  // 16-byte staging units per lane
+
+struct KendallArgs {
+    const uint32_t *pa, *pb;   // planes of A (na variables) and B (nb variables)
+    int64_t na, nb, wpad;
+    unsigned long long *count; // na x nb (row stride ldc), pair (x, y) at x * ldc + y
+    int64_t ldc;
+    int64_t tiles_a, tiles_b, ntiles;
+    int64_t split_w;           // words per workgroup (multiple of KKW)
+    int64_t id_offset, ids_this;
+};
+
+// SYM: pa == pb; tiles with J <= I only (the finish pass reads the other half
+// from the transposed position).
+template <bool SYM>
+__global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs args)
+{
+    __shared__ uint32_t lds[2][3][KKW][KBM];   // [side][plane][word][variable]
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;    // y = tx*4 + {0..3, 64..67}, x likewise with ty
+    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int64_t split = id / args.ntiles, t = id % args.ntiles;
+    int64_t I, J;
+    if (SYM) {
+        int ti, tj;
+        tri_tile(t, ti, tj);
+        I = ti;
+        J = tj;
+    } else {
+        I = t / args.tiles_b;
+        J = t % args.tiles_b;
+    }
+    const int64_t x0 = I * KBM, y0 = J * KBM;
+    const int64_t w_begin = split * args.split_w;
+    int64_t w_end = w_begin + args.split_w;
+    if (w_end > args.wpad) w_end = args.wpad;
+    if (w_begin >= w_end) return;
+
+    // staging unit u = tid + 256 q: half u & 1 (words 4 half .. +3 of the
+    // chunk), variable (u >> 1) & 127, side/plane u >> 8 (side = sp / 3)
+    const uint32_t *src[KUNITS];
+    bool ok[KUNITS];
+#pragma unroll
+    for (int q = 0; q < KUNITS; ++q) {
+        const int u = tid + KTHREADS * q;
+        const int sp = u >> 8, v = (u >> 1) & (KBM - 1), half = u & 1;
+        const int side = sp / 3, plane = sp % 3;
+        const int64_t nside = side ? args.nb : args.na;
+        const int64_t gv = (side ? y0 : x0) + v;
+        ok[q] = gv < nside;
+        src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (ok[q] ? gv : 0)) * args.wpad +
+                 half * 4;
+    }
+    uint4 stage[KUNITS];
+    auto gload = [&](int64_t w) {
+#pragma unroll
+        for (int q = 0; q < KUNITS; ++q)
+            stage[q] = ok[q] ? *reinterpret_cast<const uint4 *>(src[q] + w) : make_uint4(0, 0, 0, 0);
+    };
+
+    uint32_t cnt[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) cnt[r][c] = 0;
+
+    gload(w_begin);
+    for (int64_t w = w_begin; w < w_end; w += KKW) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KUNITS; ++q) {
+            const int u = tid + KTHREADS * q;
+            const int sp = u >> 8, v = (u >> 1) & (KBM - 1), half = u & 1;
+            uint32_t *d = &lds[sp / 3][sp % 3][half * 4][v];
+            d[0 * KBM] = stage[q].x;
+            d[1 * KBM] = stage[q].y;
+            d[2 * KBM] = stage[q].z;
+            d[3 * KBM] = stage[q].w;
+        }
+        __syncthreads();
+        if (w + KKW < w_end) gload(w + KKW);
+#pragma unroll 2
+        for (int kw = 0; kw < KKW; ++kw) {
+            uint32_t xa[3][8], yb[3][8];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const uint4 a0 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][ty * 4]);
+                const uint4 a1 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][64 + ty * 4]);
+                const uint4 b0 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][tx * 4]);
+                const uint4 b1 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][64 + tx * 4]);
+                xa[p][0] = a0.x; xa[p][1] = a0.y; xa[p][2] = a0.z; xa[p][3] = a0.w;
+                xa[p][4] = a1.x; xa[p][5] = a1.y; xa[p][6] = a1.z; xa[p][7] = a1.w;
+                yb[p][0] = b0.x; yb[p][1] = b0.y; yb[p][2] = b0.z; yb[p][3] = b0.w;
+                yb[p][4] = b1.x; yb[p][5] = b1.y; yb[p][6] = b1.z; yb[p][7] = b1.w;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    uint32_t s = cnt[r][c];
+                    s += __builtin_popcount(xa[0][r] & yb[0][c]);
+                    s += __builtin_popcount(xa[1][r] & yb[1][c]);
+                    s += __builtin_popcount(xa[2][r] & yb[2][c]);
+                    cnt[r][c] = s;
+                }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int64_t x = x0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
+        if (x >= args.na) continue;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int64_t y = y0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+            if (y >= args.nb) continue;
+            atomicAdd(args.count + x * args.ldc + y, (unsigned long long)cnt[r][c]);
+        }
+    }
+}
+
+// tau from the exact count, with the reference's arithmetic (kendall.cu:40-45);
+// count == nullptr: no sample pairs (n < 2), the count is 0.
+__global__ void kendall_finish_kernel(const unsigned long long *__restrict__ count, int64_t ldc,
+                                      int64_t na, int64_t nb, int64_t n, int sym,
+                                      double *__restrict__ tau, int64_t ldt)
+{
+    const int64_t total = na * nb;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = e % na, y = e / na;
+        const bool lower = !sym || (x / KBM >= y / KBM);
+        const unsigned long long c = count ? (lower ? count[x * ldc + y] : count[y * ldc + x]) : 0ull;
+        const float concordant = (float)c;
+        float denom = (float)n;
+        denom = (denom * (denom - 1.0f)) / 2.0f;
+        const float discordant = denom - concordant;
+        const float numer = concordant - discordant;
+        tau[y * ldt + x] = (double)numer / (double)denom;
+    }
+}
+
+unsigned grid_for(int64_t total, int threads)
+{
+    int64_t blocks = (total + threads - 1) / threads;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    return (unsigned)blocks;
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+int64_t padded_words(int64_t n)
+{
+    const int64_t w = n > 1 ? words_upto(n - 1) : 0;
+    return (w + KKW - 1) / KKW * KKW;
+}
+
+}  // namespace
+
+size_t kendall_workspace_bytes(int64_t na, int64_t nb, bool same, int64_t n)
+{
+    if (na <= 0 || nb <= 0 || n < 2) return 0;
+    const size_t wpad = (size_t)padded_words(n);
+    const size_t planes = align256(3 * (size_t)na * wpad * 4) +
+                          (same ? 0 : align256(3 * (size_t)nb * wpad * 4));
+    return planes + align256((size_t)na * (size_t)nb * 8);
+}
+
+hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
+                          int64_t ldb, bool same, int64_t n, double *tau, int64_t ldt,
+                          void *workspace, hipStream_t stream)
+{
+    if (na <= 0 || nb <= 0) return hipSuccess;
+    hipError_t e;
+    const int64_t wpad = padded_words(n);
+    char *ws = static_cast<char *>(workspace);
+    uint32_t *pa = nullptr, *pb = nullptr;
+    unsigned long long *count = nullptr;
+    if (n >= 2) {
+        pa = reinterpret_cast<uint32_t *>(ws);
+        ws += align256(3 * (size_t)na * (size_t)wpad * 4);
+        pb = pa;
+        if (!same) {
+            pb = reinterpret_cast<uint32_t *>(ws);
+            ws += align256(3 * (size_t)nb * (size_t)wpad * 4);
+        }
+        count = reinterpret_cast<unsigned long long *>(ws);
+        // zero planes (tail words) and counts with a fill kernel, not a
+        // memset node, so that a captured graph replays correctly
+        const size_t zero_floats = (size_t)(reinterpret_cast<char *>(count) - static_cast<char *>(workspace)) / 4 +
+                                   (size_t)na * (size_t)nb * 2;
+        e = launch_fill(reinterpret_cast<float *>(workspace), (int64_t)zero_floats, 0.0f, stream);
+        if (e != hipSuccess) return e;
+        {
+            const int64_t rows = na * n;
+            int64_t blocks = (rows + 3) / 4;
+            if (blocks > 65536) blocks = 65536;
+            hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                               a, na, n, lda, pa, wpad);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if (!same) {
+                const int64_t rowsb = nb * n;
+                int64_t bb = (rowsb + 3) / 4;
+                if (bb > 65536) bb = 65536;
+                hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)bb), dim3(256), 0, stream,
+                                   b, nb, n, ldb, pb, wpad);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+        }
+        KendallArgs args;
+        args.pa = pa;
+        args.pb = pb;
+        args.na = na;
+        args.nb = nb;
+        args.wpad = wpad;
+        args.count = count;
+        args.ldc = nb;
+        args.tiles_a = (na + KBM - 1) / KBM;
+        args.tiles_b = (nb + KBM - 1) / KBM;
+        args.ntiles = same ? args.tiles_a * (args.tiles_a + 1) / 2 : args.tiles_a * args.tiles_b;
+        // split the words so that about 2048 workgroups run (8 per CU)
+        int64_t splits = (2048 + args.ntiles - 1) / args.ntiles;
+        const int64_t chunks = wpad / KKW;
+        if (splits > chunks) splits = chunks;
+        if (splits < 1) splits = 1;
+        args.split_w = (chunks + splits - 1) / splits * KKW;
+        splits = (wpad + args.split_w - 1) / args.split_w;
+        const int64_t ids = args.ntiles * splits;
+        const int64_t max_ids = int64_t(1) << 23;
+        for (int64_t off = 0; off < ids; off += max_ids) {
+            args.id_offset = off;
+            args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
+            if (same)
+                hipLaunchKernelGGL(kendall_count_kernel<true>, dim3((unsigned)args.ids_this),
+                                   dim3(KTHREADS), 0, stream, args);
+            else
+                hipLaunchKernelGGL(kendall_count_kernel<false>, dim3((unsigned)args.ids_this),
+                                   dim3(KTHREADS), 0, stream, args);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    // n < 2 leaves count null: no sample pair, tau = 0 / 0
+    hipLaunchKernelGGL(kendall_finish_kernel, dim3(grid_for(na * nb, 256)), dim3(256), 0, stream,
+                       count, nb, na, nb, n, same ? 1 : 0, tau, ldt);
+    return hipGetLastError();
+}
+
+}  // namespace gt

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
         for (int rq = 0; rq < 4; ++rq) {
             // the four x rows of this lane: lx0 .. lx0 + 3
             const int lx0 = wr * 64 + a * 32 + 8 * rq + 4 * (lane >> 5);
-            float cv[2][4];
+            float cv[2][4], tv[2][4];
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
                 const int64_t x = x0 + lx0 + r4;
@@ -323,11 +323,12 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
                     // undo the scales one at a time (their product can overflow)
                     if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
                     cv[b][r4] = g;
+                    tv[b][r4] = t_score<true>(fp, g);
                     const int64_t y = y0 + ly0 + b * 32;
                     if (x < na && y < nb && (!diag || x >= y)) {
                         const int64_t o = x * ldo + y;
                         args.corr[o] = g;
-                        args.ts[o] = t_score<true>(fp, g);
+                        args.ts[o] = tv[b][r4];
                         args.pairs[o] = fp;
                     }
                 }
@@ -345,15 +346,14 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
                         *reinterpret_cast<float4 *>(args.corr + o) =
                             make_float4(cv[b][0], cv[b][1], cv[b][2], cv[b][3]);
                         *reinterpret_cast<float4 *>(args.ts + o) =
-                            make_float4(t_score<true>(fp, cv[b][0]), t_score<true>(fp, cv[b][1]),
-                                        t_score<true>(fp, cv[b][2]), t_score<true>(fp, cv[b][3]));
+                            make_float4(tv[b][0], tv[b][1], tv[b][2], tv[b][3]);
                         *reinterpret_cast<float4 *>(args.pairs + o) = make_float4(fp, fp, fp, fp);
                     } else {
 #pragma unroll
                         for (int r4 = 0; r4 < 4; ++r4) {
                             if (x + r4 >= na || (diag && x + r4 <= y)) continue;
                             args.corr[o + r4] = cv[b][r4];
-                            args.ts[o + r4] = t_score<true>(fp, cv[b][r4]);
+                            args.ts[o + r4] = tv[b][r4];
                             args.pairs[o + r4] = fp;
                         }
                     }

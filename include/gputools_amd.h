@@ -1,8 +1,8 @@
 /*
  * gputools_amd.h -- C-ABI of the MI355X (gfx950) pairwise-distance library,
  * of the hierarchical-clustering stage built on its output, and of gpuCor's
- * Pearson correlation (the same all-pairs contraction on standardised
- * variables).
+ * all-pairs correlations (Pearson: the same contraction on standardised
+ * variables; Kendall: popcounts over sample-pair bit planes).
  *
  * Two groups of entry points:
  *
@@ -151,6 +151,15 @@ void rpmcc(const int *whichObs, const float *samplesA, const int *numSamplesA,
  * t = coeff * sqrt((pairs - 2) / (1 - coeff^2)) for n entries, on the GPU,
  * without hostSignif's cap. */
 void rtestT(const float *pairs, const float *coeffs, const int *n, float *ts);
+
+/* Replaces RgpuKendall (src/rinterface.cpp:128-132, body masterKendall at
+ * src/kendall.cpp:13-55, kernel inst/cuda/kendall.cu:4-47), what
+ * gpuCor(method = "kendall") calls (R/gpuCor.R:52-55).  Variable x of X is
+ * the sampleSize floats at X + x * sampleSize.  answers: nx x ny column-major
+ * (pair (x, y) at y * nx + x), Kendall's tau as the reference defines it
+ * (sample pairs tied in both variables count as concordant). */
+void RgpuKendall(const float *X, const int *nx, const float *Y, const int *ny,
+                 const int *sampleSize, double *answers);
 
 /* Replace cuCompile / unloadPackage (src/cudaUtils.cpp:117-165, 167-195),
  * called from R/zzz.R:34-38 and :41-43.  Kernels are precompiled gfx950 code
@@ -303,6 +312,22 @@ int gt_pmcc_host(int use, int precision, const float *a_host, int64_t na, const 
                  int64_t nb, int64_t dim, float *pairs, float *corr, float *ts);
 /* gpuSignif (inst/cuda/correlation.cu:292-310) on device arrays of n entries. */
 int gt_ttest(const float *pairs, const float *corr, int64_t n, float *ts, void *stream);
+
+/* Kendall's tau of every variable of A (na x n, row stride lda >= n) with
+ * every variable of B (nb x n), the device form of masterKendall
+ * (src/kendall.cpp:13-55): tau (FP64) nb x na, row stride ldt >= na, pair
+ * (x, y) at y * ldt + x (the reference's layout).  Concordant sample pairs
+ * are counted exactly in integers; the quotient follows the reference's FP32
+ * / FP64 steps (inst/cuda/kendall.cu:40-45).  b == NULL (or b == a with the
+ * same shape) is the self case.  The workspace (gt_kendall_workspace_size)
+ * holds three bit planes of n(n-1)/2 bits per variable and the counts. */
+int gt_kendall_workspace_size(int64_t na, int64_t nb, int64_t n, int same, size_t *bytes);
+int gt_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb, int64_t ldb,
+               int64_t n, double *tau, int64_t ldt, void *workspace, size_t workspace_bytes,
+               void *stream);
+/* Host-memory form (contiguous rows; b_host == NULL for the self case). */
+int gt_kendall_host(const float *a_host, int64_t na, const float *b_host, int64_t nb, int64_t n,
+                    double *tau);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,17 @@
  *          returns: the same formula without the cap.
  * (The reference's GPU kernels are compiled with fast math; the restatement
  * uses IEEE division and square root.)
+ *
+ * oracle_kendall follows gpuKendall (inst/cuda/kendall.cu:4-47), what
+ * gpuCor(method = "kendall") returns (RgpuKendall, src/rinterface.cpp:128-132):
+ * a sample pair i < j is concordant when x_j > x_i and y_j > y_i, or
+ * x_j < x_i and y_j < y_i, or x_j == x_i and y_j == y_i (kendall.cu:18-22;
+ * comparisons with NaN are false); then, in FP32 (kendall.cu:40-45),
+ *   denom = n (n - 1) / 2, discordant = denom - concordant,
+ *   numer = concordant - discordant, tau = (double) numer / (double) denom.
+ * The count is exact (64-bit); the reference sums it in FP32, which is the
+ * same below 2^24 concordant pairs.  Output nb x na, pair (x, y) at
+ * [y * na + x] (kendall.cu:45).
  */
 #include <math.h>
 #include <stdint.h>
@@ -92,4 +103,33 @@ void oracle_signif(const float *pairs, const float *corr, int64_t n, int cap, fl
             ts[i] = cor * sqrtf(rad);
         }
     }
+}
+
+static uint64_t concordant_pairs(const float *a, const float *b, int64_t n)
+{
+    uint64_t c = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j)
+            c += (uint64_t)(((a[j] > a[i]) && (b[j] > b[i])) + ((a[j] < a[i]) && (b[j] < b[i])) +
+                            ((a[j] == a[i]) && (b[j] == b[i])));
+    return c;
+}
+
+int oracle_kendall(const float *a, int64_t na, const float *b, int64_t nb, int64_t n, double *tau,
+                   uint64_t *count, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t y = 0; y < nb; ++y)
+        for (int64_t x = 0; x < na; ++x) {
+            const uint64_t c = concordant_pairs(a + x * n, b + y * n, n);
+            const float concordant = (float)c;
+            float denom = (float)n;
+            denom = (denom * (denom - 1.f)) / 2.f;
+            const float discordant = denom - concordant;
+            const float numer = concordant - discordant;
+            tau[y * na + x] = (double)numer / (double)denom;
+            if (count) count[y * na + x] = c;
+        }
+    return 0;
 }

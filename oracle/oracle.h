@@ -50,6 +50,9 @@ int oracle_hclust_format(int len, const int *sub, const int *sup, int *merge, in
 int oracle_pmcc(int use, const float *a, int64_t na, const float *b, int64_t nb, int64_t dim,
                 double *pairs, double *corr, int nthreads);
 void oracle_signif(const float *pairs, const float *corr, int64_t n, int cap, float *ts);
+/* Kendall's tau as gpuKendall defines it; tau and count nb x na ([y * na + x]). */
+int oracle_kendall(const float *a, int64_t na, const float *b, int64_t nb, int64_t n, double *tau,
+                   uint64_t *count, int nthreads);
 
 double oracle_mds_stress(const float *y, int64_t n, int64_t q, const double *delta,
                          double *dy, int nthreads);

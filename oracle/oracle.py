@@ -10,7 +10,8 @@ differ.  It also restates the clustering stage of gpuDistClust / gpuHclust
 (src/hcluster.cpp:40-201, inst/cuda/hcluster.cu, src/rinterface.cpp:241-292)
 in the reference's own FP32 arithmetic; see hclust_oracle.c.  And it restates
 gpuCor's Pearson correlation (src/correlation.cpp:51-160,
-inst/cuda/correlation.cu) in FP64 with the t-scores in FP32; see cor_oracle.c.
+inst/cuda/correlation.cu) in FP64 with the t-scores in FP32, and its Kendall
+tau (inst/cuda/kendall.cu); see cor_oracle.c.
 """
 from __future__ import annotations
 
@@ -79,6 +80,9 @@ def lib():
         L.oracle_pmcc.restype = ctypes.c_int
         L.oracle_signif.argtypes = [_f32p, _f32p, _i64, ctypes.c_int, _f32p]
         L.oracle_signif.restype = None
+        L.oracle_kendall.argtypes = [_f32p, _i64, _f32p, _i64, _i64, _f64p, ctypes.c_void_p,
+                                     ctypes.c_int]
+        L.oracle_kendall.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -231,3 +235,21 @@ def signif(pairs, corr, cap=True) -> np.ndarray:
     ts = np.empty_like(c)
     lib().oracle_signif(p, c, c.shape[0], 1 if cap else 0, ts)
     return ts.reshape(np.shape(corr))
+
+
+def kendall(a, b=None, nthreads=0):
+    """Kendall's tau as gputools' gpuKendall defines it, for every row
+    (variable) of ``a`` against every row of ``b`` (default ``a``).  Returns
+    (tau, count): nb x na arrays (entry (y, x)), tau FP64, count the exact
+    number of concordant sample pairs (uint64)."""
+    a = _f32(a)
+    b = a if b is None else _f32(b)
+    if a.ndim != 2 or b.ndim != 2 or a.shape[1] != b.shape[1]:
+        raise ValueError("a and b must be 2-D with the same number of samples")
+    na, n = a.shape
+    nb = b.shape[0]
+    tau = np.empty(na * nb, dtype=np.float64)
+    count = np.empty(na * nb, dtype=np.uint64)
+    if lib().oracle_kendall(a, na, b, nb, n, tau, count.ctypes.data, nthreads) != 0:
+        raise ValueError("oracle_kendall rejected its arguments")
+    return tau.reshape(nb, na), count.reshape(nb, na)

@@ -181,3 +181,103 @@ def test_pmcc_workspace_sizes():
     # three BF16 terms of the samples padded to 32, plus one scale per variable
     assert b.value >= 3 * 150 * 32 * 2 + 150 * 4
     assert L.gt_pmcc_workspace_size(0, 1, 0, 50, 30, 0, ctypes.byref(b)) == 0 and b.value == 0
+
+
+# ------------------------------- Kendall -----------------------------------
+# Pins of oracle_kendall (gpuKendall's definition: pairs tied in both
+# variables count as concordant, NaN comparisons are false):
+#   * scipy.stats.kendalltau on tie-free data, where that definition and
+#     scipy's tau-b coincide with (C - D) / (n (n - 1) / 2);
+#   * closed forms: identical / reversed orderings (+1 / -1), a constant
+#     variable against distinct values (no concordant pair: -1) and against
+#     another constant (every pair tied in both: +1), n < 2 (0 / 0);
+#   * brute-force enumeration of the three tests on tiny inputs with ties and
+#     NaN, written independently of the oracle's loop order.
+
+@pytest.mark.parametrize("na,nb,n", [(5, 10, 30), (3, 2, 200), (1, 1, 2)])
+def test_kendall_oracle_matches_scipy_without_ties(na, nb, n):
+    stats = pytest.importorskip("scipy.stats")
+    rng = np.random.default_rng(na + nb + n)
+    a = rng.standard_normal((na, n)).astype(np.float32)
+    b = rng.standard_normal((nb, n)).astype(np.float32)
+    tau, count = orc.kendall(a, b)
+    for y in range(nb):
+        for x in range(na):
+            want = stats.kendalltau(a[x], b[y]).statistic
+            assert abs(tau[y, x] - want) < 1e-12
+
+
+def test_kendall_oracle_closed_forms():
+    n = 40
+    up = np.arange(n, dtype=np.float32)[None]
+    tau, count = orc.kendall(up, np.concatenate([up, -up, np.zeros_like(up)]))
+    assert tau[0, 0] == 1.0 and tau[1, 0] == -1.0 and tau[2, 0] == -1.0
+    assert count[0, 0] == n * (n - 1) // 2 and count[2, 0] == 0
+    const = np.full((1, n), 3.0, np.float32)
+    assert orc.kendall(const, const)[0][0, 0] == 1.0
+    # x = (1, 2, 3), y = (1, 3, 2): pairs (1,2) and (1,3) concordant, (2,3) not
+    t, c = orc.kendall(np.float32([[1, 2, 3]]), np.float32([[1, 3, 2]]))
+    assert c[0, 0] == 2 and abs(t[0, 0] - 1 / 3) < 1e-15
+    for n0 in (0, 1):
+        t0, _ = orc.kendall(np.zeros((2, n0), np.float32), np.zeros((1, n0), np.float32))
+        assert np.isnan(t0).all()
+
+
+def test_kendall_oracle_brute_force_with_ties_and_nan():
+    import itertools
+    rng = np.random.default_rng(9)
+    for _ in range(30):
+        n = int(rng.integers(2, 12))
+        a = rng.integers(0, 3, n).astype(np.float32)
+        b = rng.integers(0, 3, n).astype(np.float32)
+        a[rng.random(n) < 0.15] = np.nan
+        c = 0
+        for i, j in itertools.combinations(range(n), 2):
+            sa, sb = a[j] - a[i], b[j] - b[i]   # NaN when either value is NaN
+            if (sa > 0 and sb > 0) or (sa < 0 and sb < 0) or (sa == 0 and sb == 0):
+                c += 1
+        tau, count = orc.kendall(a[None], b[None])
+        assert count[0, 0] == c
+        k = n * (n - 1) / 2
+        assert abs(tau[0, 0] - (2 * c - k) / k) < 1e-12
+
+
+def test_kendall_oracle_rounds_large_counts_in_fp32():
+    """Above 2^24 concordant pairs the FP32 steps round (n = 6000: 17,997,000
+    sample pairs; identical variables are concordant on all of them)."""
+    n = 6000
+    x = np.arange(n, dtype=np.float32)[None]
+    tau, count = orc.kendall(x, x)
+    assert count[0, 0] == n * (n - 1) // 2 and tau[0, 0] == 1.0
+    y = x.copy()
+    y[0, :1002] = y[0, :1002][::-1]   # 501,501 discordant pairs: an odd count above 2^24
+    tau, count = orc.kendall(x, y)
+    c = int(count[0, 0])
+    cf = np.float32(c)
+    denom = np.float32(np.float32(n) * np.float32(n - 1)) / np.float32(2)
+    numer = cf - (denom - cf)
+    assert tau[0, 0] == float(numer) / float(denom)
+    assert c > 2 ** 24 and c % 2 == 1 and int(cf) != c   # the count itself rounds
+
+
+def test_gpucor_kendall_warns_on_pairwise_use():
+    with pytest.warns(UserWarning, match="NA handling for Kendall"):
+        with pytest.raises(Exception):
+            # the warning precedes the library call, which has no GPU here
+            gt.gpuCor(np.zeros((4, 2)), method="kendall", use="pairwise.complete.obs")
+
+
+def test_capi_rejects_bad_kendall_arguments():
+    L = _lib.lib()
+    buf = (ctypes.c_float * 16)()
+    p = ctypes.addressof(buf)
+    assert L.gt_kendall(p, 2, 1, p, 2, 2, 2, p, 2, None, 0, None) == _lib.GT_ERR_INVALID
+    assert L.gt_kendall(p, 2, 2, p, 2, 2, 2, p, 1, None, 0, None) == _lib.GT_ERR_INVALID
+    assert L.gt_kendall(p, 2, 2, p, 2, 2, 2, None, 2, None, 0, None) == _lib.GT_ERR_INVALID
+    assert L.gt_kendall(p, 2, 2, p, 2, 2, (1 << 22) + 1, p, 2, None, 0, None) == _lib.GT_ERR_INVALID
+    assert L.gt_kendall(p, 2, 2, p, 2, 2, 2, p, 2, None, 0, None) == _lib.GT_ERR_WORKSPACE
+    b = ctypes.c_size_t()
+    assert L.gt_kendall_workspace_size(4, 4, 1, 0, ctypes.byref(b)) == 0 and b.value == 0
+    assert L.gt_kendall_workspace_size(4, 4, 100, 1, ctypes.byref(b)) == 0
+    # 3 planes x 4 variables x ceil-padded words, plus 4 x 4 64-bit counts
+    assert b.value >= 3 * 4 * (100 * 99 // 2) // 32 * 4 + 4 * 4 * 8

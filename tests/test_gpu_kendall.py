@@ -1,0 +1,92 @@
+"""GPU parity of gpuCor(method = "kendall") (RgpuKendall, gt_kendall).
+
+The count of concordant sample pairs is exact on both sides and the final
+FP32 / FP64 steps are the same, so tau is compared bit for bit with the
+oracle (oracle/cor_oracle.c, pinned in test_cor_cpu.py), NaN with NaN.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+torch = pytest.importorskip("torch")
+
+import gputools_amd as gt  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(got, want):
+    got = np.asarray(got, np.float64)
+    nan = np.isnan(want)
+    np.testing.assert_array_equal(np.isnan(got), nan)
+    np.testing.assert_array_equal(got[~nan].view(np.uint64), want[~nan].view(np.uint64))
+
+
+def check_gpucor(x, y=None):
+    res = gt.gpuCor(x, y, method="kendall")
+    a = np.ascontiguousarray(x.T, dtype=np.float32)
+    b = a if y is None else np.ascontiguousarray(y.T, dtype=np.float32)
+    tau, _ = orc.kendall(a, b, nthreads=8)
+    assert_same(res["coefficients"], tau.T)        # R's nx x ny matrix
+    assert (res["pairs"] == x.shape[0]).all()
+    return res
+
+
+@pytest.mark.parametrize("n,nx,ny", [(30, 5, 10), (200, 130, 70), (33, 3, 4), (65, 129, 1),
+                                     (97, 2, 257), (2, 3, 3), (1, 2, 2)])
+def test_kendall_matches_oracle_continuous(n, nx, ny):
+    rng = np.random.default_rng(n + nx + ny)
+    x = rng.standard_normal((n, nx)).astype(np.float32)
+    y = rng.standard_normal((n, ny)).astype(np.float32)
+    print("case kendall continuous n=%d nx=%d ny=%d" % (n, nx, ny), flush=True)
+    check_gpucor(x, y)
+
+
+def test_kendall_ties_and_nan():
+    rng = np.random.default_rng(3)
+    x = rng.integers(0, 4, (150, 40)).astype(np.float32)
+    y = rng.integers(0, 3, (150, 25)).astype(np.float32)
+    x[rng.random(x.shape) < 0.05] = np.nan
+    y[:, 0] = 1.0                                   # a constant variable
+    x[:, 1] = np.nan                                # no comparable sample at all
+    print("case kendall ties/nan", flush=True)
+    res = check_gpucor(x, y)
+    assert (res["coefficients"][1] == -1.0).all()   # nothing concordant
+
+
+def test_kendall_self_is_symmetric():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((300, 300)).astype(np.float32)
+    x[:, :20] = np.round(x[:, :20])                 # some ties
+    print("case kendall self 300x300", flush=True)
+    res = check_gpucor(x)
+    c = res["coefficients"]
+    np.testing.assert_array_equal(c, c.T)
+    assert (np.diag(c) == 1.0).all()
+
+
+def test_kendall_count_above_2_24():
+    """6000 samples: 17,997,000 sample pairs, counts above 2^24 whose FP32
+    conversion rounds (the oracle follows the same steps)."""
+    n = 6000
+    rng = np.random.default_rng(5)
+    base = np.arange(n, dtype=np.float32)
+    cols = [base, base.copy(), rng.permutation(base), base + rng.standard_normal(n).astype(np.float32) * 50]
+    cols[1][:1002] = cols[1][:1002][::-1]
+    x = np.stack(cols, axis=1)
+    print("case kendall n=6000", flush=True)
+    check_gpucor(x)
+
+
+def test_device_kendall_strided():
+    rng = np.random.default_rng(6)
+    base = torch.from_numpy(rng.standard_normal((140, 260)).astype(np.float32)).cuda()
+    a = base[:, :250]
+    b = base[:60, 5:255]
+    out = torch.full((60, 144), 7.0, dtype=torch.float64, device="cuda")[:, :140]
+    print("case kendall device strided", flush=True)
+    gt.kendall(a, b, out=out)
+    torch.cuda.synchronize()
+    tau, _ = orc.kendall(a.cpu().numpy(), b.cpu().numpy(), nthreads=8)
+    assert_same(out.cpu().numpy(), tau)
