@@ -52,7 +52,7 @@ def parse():
                     help="skip the secondary paths (other methods, batched, backward, MDS, ...)")
     ap.add_argument("--paths", default=None,
                     help="comma list of secondary path groups to time (default: all of "
-                         "methods,batched,backward,small,hclust,pmcc)")
+                         "methods,batched,backward,small,hclust,pmcc,kendall)")
     ap.add_argument("--paths-only", action="store_true",
                     help="time only the selected secondary paths (no headline line; for "
                          "profiling them)")
@@ -257,8 +257,28 @@ def path_pmcc(torch, gt, wl, steps, warmup, out):
     out["pmcc_self_8192x1024"] = cor
 
 
+def path_kendall(torch, gt, wl, steps, warmup, out):
+    # gpuCor(x, method = "kendall"): 1024 variables x 2048 samples N(0,1),
+    # self case.  Algorithmic work of the count: per unique pair of variables
+    # and 32 sample pairs, 3 ANDs and 3 popcount-accumulates (32-bit integer
+    # VALU lane operations; 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s).
+    nv, size = 1024, 2048
+    a = torch.from_numpy(wl.normal(nv, size, seed=8)).cuda()
+    tau = torch.empty((nv, nv), dtype=torch.float64, device="cuda")
+    ms = _timed(lambda: gt.kendall(a, out=tau), steps, warmup, torch)
+    q, r = divmod(size - 1, 32)
+    words = 16 * q * (q + 1) + r * (q + 1)
+    ops = 6.0 * words * nv * (nv + 1) / 2
+    tops = ops / (ms * 1e-3) / 1e12
+    out["kendall_self_1024x2048"] = {
+        "ms": round(ms, 4), "pairs_per_s": round(nv * nv / (ms * 1e-3), 1),
+        "sample_pairs_per_s": round(nv * (nv + 1) / 2 * size * (size - 1) / 2 / (ms * 1e-3), 1),
+        "roofline": {"bound": "valu", "achieved": round(tops, 2), "peak": 39.3,
+                     "unit": "T int32 lane-ops/s", "frac": round(tops / 39.3, 4)}}
+
+
 PATHS = {"methods": path_methods, "batched": path_batched, "backward": path_backward,
-         "small": path_small, "hclust": path_hclust, "pmcc": path_pmcc}
+         "small": path_small, "hclust": path_hclust, "pmcc": path_pmcc, "kendall": path_kendall}
 
 
 def secondary_paths(torch, gt, wl, select=None, steps=3, warmup=1):

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
                                                              uint32_t *__restrict__ planes,
                                                              int64_t wpad)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t rows = nvar * n, wtot = words_upto(n - 1);
     const float qnan = __builtin_nanf("");
     for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < rows; r += (int64_t)gridDim.x * 4) {
@@ -72,9 +73,9 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
         for (int64_t t = 0; t * 64 < m; ++t) {
             const int64_t j = i + 1 + t * 64 + lane;
             const float xj = j < n ? xv[j] : qnan;
-            const unsigned long long gt = __ballot(xj > xi);
-            const unsigned long long lt = __ballot(xj < xi);
-            const unsigned long long eq = __ballot(xj == xi);
+            const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj > xi);
+            const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj < xi);
+            const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj == xi);
             const unsigned long long mine = plane == 0 ? gt : (plane == 1 ? lt : eq);
             if (lane < 6 && 2 * t + half < wr) dst[2 * t] = (uint32_t)(mine >> (32 * half));
         }
@@ -186,10 +187,16 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
             for (int r = 0; r < 8; ++r)
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
+                    // one AND and one v_bcnt_u32_b32 (popcount plus
+                    // accumulator) per plane: the empty asm keeps the compiler
+                    // from regrouping the three sums into an extra v_add3
                     uint32_t s = cnt[r][c];
                     s += __builtin_popcount(xa[0][r] & yb[0][c]);
+                    asm("" : "+v"(s));
                     s += __builtin_popcount(xa[1][r] & yb[1][c]);
+                    asm("" : "+v"(s));
                     s += __builtin_popcount(xa[2][r] & yb[2][c]);
+                    asm("" : "+v"(s));
                     cnt[r][c] = s;
                 }
         }
