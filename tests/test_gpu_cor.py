@@ -45,6 +45,7 @@ def _check(res, x, y, use):
     fin = np.isfinite(corr)
     np.testing.assert_array_equal(np.isfinite(got), fin)
     err = np.abs(got[fin] - corr[fin]).max() if fin.any() else 0.0
+    print("max abs error %.3g" % err, flush=True)
     assert err <= TOL, err
     ts = orc.signif(res["pairs"].astype(np.float32), got.astype(np.float32))
     assert_bits_equal(res["ts"], ts)
@@ -89,6 +90,7 @@ def test_device_pmcc_all_precisions(precision, self_case):
     torch.cuda.synchronize()
     ref_p, ref_c = orc.pmcc(a, a if self_case else b, nthreads=8)
     got = corr.cpu().numpy()
+    print("max abs error %.3g" % np.abs(got - ref_c).max(), flush=True)
     assert np.abs(got - ref_c).max() <= TOL
     assert (pairs.cpu().numpy() == 700).all()
     want_ts = orc.signif(pairs.cpu().numpy(), got)
@@ -181,6 +183,7 @@ def test_gpucor_larger_self_sampled_rows():
     res = gt.gpuCor(x)
     a = np.ascontiguousarray(x.T)
     pairs, corr = orc.pmcc(a[:96], a, nthreads=16)
+    print("max abs error %.3g" % np.abs(res["coefficients"][:96] - corr).max(), flush=True)
     assert np.abs(res["coefficients"][:96] - corr).max() <= TOL
     np.testing.assert_array_equal(res["coefficients"], res["coefficients"].T)
 
