@@ -8,7 +8,9 @@ list (``coefficients``, ``ts`` and ``pairs`` as nx x ny matrices, values
 widened to double as ``.C`` returns them).  The reference matches ``method``
 and ``use`` with ``pmatch(..., -1)``, whose third argument is ``nomatch``: an
 unknown or ambiguous name yields -1, so every failed match stops with
-"ambiguous correlation method" (R/gpuCor.R:15-31); that is kept.  Unlike the
+"ambiguous correlation method" (R/gpuCor.R:15-31); that is kept.  Pearson
+is called with NAOK = TRUE (NaN is data), Kendall without it (R refuses
+NA/NaN/Inf before the call).  Unlike the
 reference, y must have as many rows as x (the reference reads past the end
 of y otherwise).
 
@@ -93,6 +95,11 @@ def gpuCor(x, y=None, use: str = "everything", method: str = "pearson") -> dict:
     if use != "everything":
         warnings.warn("NA handling for Kendall's is not yet supported. Defaulting to using "
                       "everything. Sorry for any inconvenience.")
+    # .C("RgpuKendall", ...) runs without NAOK = TRUE, so R itself refuses
+    # NA/NaN/Inf in x (argument 1) and y (argument 3)
+    for arr, arg in ((a, 1), (b, 3)):
+        if not np.all(np.isfinite(arr)):
+            raise ValueError("NA/NaN/Inf in foreign function call (arg %d)" % arg)
     result = np.zeros(max(n, 1), np.float64)
     fp = ctypes.POINTER(ctypes.c_float)
     cnx, cny, csz = ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(size)

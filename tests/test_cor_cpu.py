@@ -260,11 +260,15 @@ def test_kendall_oracle_rounds_large_counts_in_fp32():
     assert c > 2 ** 24 and c % 2 == 1 and int(cf) != c   # the count itself rounds
 
 
-def test_gpucor_kendall_warns_on_pairwise_use():
+def test_gpucor_kendall_warns_on_pairwise_use_and_refuses_na():
+    x = np.zeros((4, 2))
+    x[1, 1] = np.nan
     with pytest.warns(UserWarning, match="NA handling for Kendall"):
-        with pytest.raises(Exception):
-            # the warning precedes the library call, which has no GPU here
-            gt.gpuCor(np.zeros((4, 2)), method="kendall", use="pairwise.complete.obs")
+        # .C without NAOK: R refuses NA/NaN/Inf before the library is called
+        with pytest.raises(ValueError, match="NA/NaN/Inf in foreign function call"):
+            gt.gpuCor(x, method="kendall", use="pairwise.complete.obs")
+    with pytest.raises(ValueError, match=r"\(arg 3\)"):
+        gt.gpuCor(np.zeros((4, 2)), np.full((4, 1), np.inf), method="kendall")
 
 
 def test_capi_rejects_bad_kendall_arguments():

@@ -43,16 +43,29 @@ def test_kendall_matches_oracle_continuous(n, nx, ny):
     check_gpucor(x, y)
 
 
-def test_kendall_ties_and_nan():
+def test_kendall_ties():
     rng = np.random.default_rng(3)
     x = rng.integers(0, 4, (150, 40)).astype(np.float32)
     y = rng.integers(0, 3, (150, 25)).astype(np.float32)
-    x[rng.random(x.shape) < 0.05] = np.nan
     y[:, 0] = 1.0                                   # a constant variable
-    x[:, 1] = np.nan                                # no comparable sample at all
-    print("case kendall ties/nan", flush=True)
-    res = check_gpucor(x, y)
-    assert (res["coefficients"][1] == -1.0).all()   # nothing concordant
+    print("case kendall ties", flush=True)
+    check_gpucor(x, y)
+
+
+def test_kendall_nan_through_the_native_api():
+    """R refuses NaN before calling RgpuKendall; the native API defines it:
+    every comparison with NaN is false, so such sample pairs are discordant."""
+    rng = np.random.default_rng(13)
+    a = rng.integers(0, 4, (40, 150)).astype(np.float32)
+    b = rng.integers(0, 3, (25, 150)).astype(np.float32)
+    a[rng.random(a.shape) < 0.05] = np.nan
+    a[1] = np.nan                                   # no comparable sample at all
+    print("case kendall nan (device api)", flush=True)
+    tau = gt.kendall(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    torch.cuda.synchronize()
+    want, _ = orc.kendall(a, b, nthreads=8)
+    assert_same(tau.cpu().numpy(), want)
+    assert (tau.cpu().numpy()[:, 1] == -1.0).all()  # nothing concordant
 
 
 def test_kendall_self_is_symmetric():
