@@ -438,33 +438,65 @@ __global__ __launch_bounds__(WTHREADS) void pmcc_pairwise_kernel(PairwiseArgs ar
 #pragma unroll
         for (int j = 0; j < 4; ++j) cnt[i][j] = s1[i][j] = s2[i][j] = 0.0f;
 
-    // pass 1: count, sum x, sum y over the pair's samples
+    // pass 1: count, sum x, sum y over the pair's samples.  A chunk without
+    // Inf uses products with 0/1 masks (x * 0 = 0 for finite x): one fma per
+    // sum instead of a select and an add, with the same sums bit for bit; a
+    // chunk with an Inf selects.
     gload(0);
     for (int64_t kb = 0; kb < dim; kb += WBK) {
         __syncthreads();
         stage();
-        __syncthreads();
+        bool inf_here = false;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) inf_here = inf_here || __builtin_isinf(ra[q]) || __builtin_isinf(rb[q]);
+        const bool any_inf = __syncthreads_or(inf_here) != 0;
         if (kb + WBK < dim) gload(kb + WBK);
         float p1[4][4], p2[4][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) p1[i][j] = p2[i][j] = 0.0f;
+        if (!any_inf) {
 #pragma unroll 4
-        for (int kc = 0; kc < WBK; ++kc) {
-            const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
-            const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
-            const float av[4] = {a4.x, a4.y, a4.z, a4.w};
-            const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+            for (int kc = 0; kc < WBK; ++kc) {
+                const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
+                const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
+                const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+                float am[4], a0[4], bm[4], b0[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
-                    cnt[i][j] += m ? 1.0f : 0.0f;
-                    p1[i][j] += m ? av[i] : 0.0f;
-                    p2[i][j] += m ? bv[j] : 0.0f;
+                for (int i = 0; i < 4; ++i) {
+                    am[i] = av[i] == av[i] ? 1.0f : 0.0f;
+                    a0[i] = av[i] == av[i] ? av[i] : 0.0f;
+                    bm[i] = bv[i] == bv[i] ? 1.0f : 0.0f;
+                    b0[i] = bv[i] == bv[i] ? bv[i] : 0.0f;
                 }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        cnt[i][j] = fmaf(am[i], bm[j], cnt[i][j]);
+                        p1[i][j] = fmaf(a0[i], bm[j], p1[i][j]);
+                        p2[i][j] = fmaf(b0[j], am[i], p2[i][j]);
+                    }
+            }
+        } else {
+#pragma unroll 4
+            for (int kc = 0; kc < WBK; ++kc) {
+                const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
+                const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
+                const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
+                        cnt[i][j] += m ? 1.0f : 0.0f;
+                        p1[i][j] += m ? av[i] : 0.0f;
+                        p2[i][j] += m ? bv[j] : 0.0f;
+                    }
+            }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
