@@ -42,7 +42,7 @@ constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
 constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
 constexpr int GTHREADS = 1024;  // 16 waves in a 4 (A) x 4 (B) grid, each 64 x 64 pairs
-constexpr int GSUPER = 2;       // super-tile side in tiles (L2 locality)
+constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
 constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
 constexpr int FCAP = 1024;      // flagged pairs per fix-up pass

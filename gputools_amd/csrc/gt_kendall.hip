@@ -20,7 +20,9 @@
 //   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
 //                      + popc(Ex & Ey)
 // exactly, 6 VALU operations per 32 sample pairs per pair of variables (AND
-// and v_bcnt_u32_b32, which adds the count to an accumulator).  The count
+// and v_bcnt_u32_b32, which adds the count to an accumulator), or 4 where no
+// variable on one side of the tile has a tie (its E planes are all zero:
+// continuous data).  The count
 // kernel tiles 128 x 128 pairs of variables over 256 lanes (8 x 8 each),
 // stages word chunks of both sides through LDS and splits the word range over
 // workgroups, adding its counts with 64-bit atomics.  A last pass applies the
@@ -49,11 +51,13 @@ __host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
 // Plane p (0 = P, 1 = N, 2 = E) of variable v, word w at
 // planes[(p * nvar + v) * wpad + w]; row i starts at word
 // words_upto(n - 1) - words_upto(n - 1 - i).  Words past the last row are
-// zero (cleared before this pass).
+// zero (cleared before this pass).  ties[v] (cleared before) becomes 1 when
+// any sample pair of v is tied, i.e. its E plane has a set bit.
 __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__restrict__ x,
                                                              int64_t nvar, int64_t n, int64_t ldx,
                                                              uint32_t *__restrict__ planes,
-                                                             int64_t wpad)
+                                                             int64_t wpad,
+                                                             uint32_t *__restrict__ ties)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -78,6 +82,8 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
             const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj == xi);
             const unsigned long long mine = plane == 0 ? gt : (plane == 1 ? lt : eq);
             if (lane < 6 && 2 * t + half < wr) dst[2 * t] = (uint32_t)(mine >> (32 * half));
+            // a tie anywhere in the variable: its E plane is not all zero
+            if (lane == 0 && eq != 0ull) ties[v] = 1u;
         }
     }
 }
@@ -97,8 +103,45 @@ struct KendallArgs {
     int64_t ldc;
     int64_t tiles_a, tiles_b, ntiles;
     int64_t split_w;           // words per workgroup (multiple of KKW)
+    const uint32_t *ties_a, *ties_b;   // per variable: 1 if its E plane has a set bit
     int64_t id_offset, ids_this;
 };
+
+// Accumulate one staged chunk into the 8 x 8 counts of this lane: per plane
+// and pair one AND and one v_bcnt_u32_b32 (popcount plus accumulator); the
+// empty asm keeps the compiler from regrouping the sums into an extra v_add3.
+template <int NP>
+__device__ __forceinline__ void count_chunk(const uint32_t (&lds)[2][3][KKW][KBM], int tx, int ty,
+                                            uint32_t (&cnt)[8][8])
+{
+#pragma unroll 2
+    for (int kw = 0; kw < KKW; ++kw) {
+        uint32_t xa[NP][8], yb[NP][8];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const uint4 a0 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][ty * 4]);
+            const uint4 a1 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][64 + ty * 4]);
+            const uint4 b0 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][tx * 4]);
+            const uint4 b1 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][64 + tx * 4]);
+            xa[p][0] = a0.x; xa[p][1] = a0.y; xa[p][2] = a0.z; xa[p][3] = a0.w;
+            xa[p][4] = a1.x; xa[p][5] = a1.y; xa[p][6] = a1.z; xa[p][7] = a1.w;
+            yb[p][0] = b0.x; yb[p][1] = b0.y; yb[p][2] = b0.z; yb[p][3] = b0.w;
+            yb[p][4] = b1.x; yb[p][5] = b1.y; yb[p][6] = b1.z; yb[p][7] = b1.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                uint32_t s = cnt[r][c];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    s += __builtin_popcount(xa[p][r] & yb[p][c]);
+                    asm("" : "+v"(s));
+                }
+                cnt[r][c] = s;
+            }
+    }
+}
 
 // SYM: pa == pb; tiles with J <= I only (the finish pass reads the other half
 // from the transposed position).
@@ -126,6 +169,22 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
     if (w_end > args.wpad) w_end = args.wpad;
     if (w_begin >= w_end) return;
 
+    // The E planes only matter when a variable on each side has a tie; for
+    // tie-free tiles (continuous data) they are all zero and are skipped:
+    // two planes, 4 instead of 6 operations per word.
+    int tie = 0;
+    if (tid < KBM) {
+        const int64_t v = x0 + tid;
+        tie = v < args.na ? (int)args.ties_a[v] : 0;
+    }
+    const bool tie_x = __syncthreads_or(tie) != 0;
+    tie = 0;
+    if (tid < KBM) {
+        const int64_t v = y0 + tid;
+        tie = v < args.nb ? (int)args.ties_b[v] : 0;
+    }
+    const bool use_e = tie_x && (__syncthreads_or(tie) != 0);
+
     // staging unit u = tid + 256 q: half u & 1 (words 4 half .. +3 of the
     // chunk), variable (u >> 1) & 127, side/plane u >> 8 (side = sp / 3)
     const uint32_t *src[KUNITS];
@@ -137,7 +196,7 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const int side = sp / 3, plane = sp % 3;
         const int64_t nside = side ? args.nb : args.na;
         const int64_t gv = (side ? y0 : x0) + v;
-        ok[q] = gv < nside;
+        ok[q] = gv < nside && (plane < 2 || use_e);
         src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (ok[q] ? gv : 0)) * args.wpad +
                  half * 4;
     }
@@ -169,37 +228,10 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         }
         __syncthreads();
         if (w + KKW < w_end) gload(w + KKW);
-#pragma unroll 2
-        for (int kw = 0; kw < KKW; ++kw) {
-            uint32_t xa[3][8], yb[3][8];
-#pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                const uint4 a0 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][ty * 4]);
-                const uint4 a1 = *reinterpret_cast<const uint4 *>(&lds[0][p][kw][64 + ty * 4]);
-                const uint4 b0 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][tx * 4]);
-                const uint4 b1 = *reinterpret_cast<const uint4 *>(&lds[1][p][kw][64 + tx * 4]);
-                xa[p][0] = a0.x; xa[p][1] = a0.y; xa[p][2] = a0.z; xa[p][3] = a0.w;
-                xa[p][4] = a1.x; xa[p][5] = a1.y; xa[p][6] = a1.z; xa[p][7] = a1.w;
-                yb[p][0] = b0.x; yb[p][1] = b0.y; yb[p][2] = b0.z; yb[p][3] = b0.w;
-                yb[p][4] = b1.x; yb[p][5] = b1.y; yb[p][6] = b1.z; yb[p][7] = b1.w;
-            }
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    // one AND and one v_bcnt_u32_b32 (popcount plus
-                    // accumulator) per plane: the empty asm keeps the compiler
-                    // from regrouping the three sums into an extra v_add3
-                    uint32_t s = cnt[r][c];
-                    s += __builtin_popcount(xa[0][r] & yb[0][c]);
-                    asm("" : "+v"(s));
-                    s += __builtin_popcount(xa[1][r] & yb[1][c]);
-                    asm("" : "+v"(s));
-                    s += __builtin_popcount(xa[2][r] & yb[2][c]);
-                    asm("" : "+v"(s));
-                    cnt[r][c] = s;
-                }
-        }
+        if (use_e)
+            count_chunk<3>(lds, tx, ty, cnt);
+        else
+            count_chunk<2>(lds, tx, ty, cnt);
     }
 
 #pragma unroll
@@ -260,7 +292,8 @@ size_t kendall_workspace_bytes(int64_t na, int64_t nb, bool same, int64_t n)
     const size_t wpad = (size_t)padded_words(n);
     const size_t planes = align256(3 * (size_t)na * wpad * 4) +
                           (same ? 0 : align256(3 * (size_t)nb * wpad * 4));
-    return planes + align256((size_t)na * (size_t)nb * 8);
+    const size_t ties = align256((size_t)na * 4) + (same ? 0 : align256((size_t)nb * 4));
+    return planes + ties + align256((size_t)na * (size_t)nb * 8);
 }
 
 hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
@@ -271,7 +304,7 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
     hipError_t e;
     const int64_t wpad = padded_words(n);
     char *ws = static_cast<char *>(workspace);
-    uint32_t *pa = nullptr, *pb = nullptr;
+    uint32_t *pa = nullptr, *pb = nullptr, *ta = nullptr, *tb = nullptr;
     unsigned long long *count = nullptr;
     if (n >= 2) {
         pa = reinterpret_cast<uint32_t *>(ws);
@@ -281,8 +314,15 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
             pb = reinterpret_cast<uint32_t *>(ws);
             ws += align256(3 * (size_t)nb * (size_t)wpad * 4);
         }
+        ta = reinterpret_cast<uint32_t *>(ws);
+        ws += align256((size_t)na * 4);
+        tb = ta;
+        if (!same) {
+            tb = reinterpret_cast<uint32_t *>(ws);
+            ws += align256((size_t)nb * 4);
+        }
         count = reinterpret_cast<unsigned long long *>(ws);
-        // zero planes (tail words) and counts with a fill kernel, not a
+        // zero planes (tail words), tie flags and counts with a fill kernel, not a
         // memset node, so that a captured graph replays correctly
         const size_t zero_floats = (size_t)(reinterpret_cast<char *>(count) - static_cast<char *>(workspace)) / 4 +
                                    (size_t)na * (size_t)nb * 2;
@@ -293,14 +333,14 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
             int64_t blocks = (rows + 3) / 4;
             if (blocks > 65536) blocks = 65536;
             hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
-                               a, na, n, lda, pa, wpad);
+                               a, na, n, lda, pa, wpad, ta);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if (!same) {
                 const int64_t rowsb = nb * n;
                 int64_t bb = (rowsb + 3) / 4;
                 if (bb > 65536) bb = 65536;
                 hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)bb), dim3(256), 0, stream,
-                                   b, nb, n, ldb, pb, wpad);
+                                   b, nb, n, ldb, pb, wpad, tb);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         }
@@ -312,6 +352,8 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
         args.wpad = wpad;
         args.count = count;
         args.ldc = nb;
+        args.ties_a = ta;
+        args.ties_b = tb;
         args.tiles_a = (na + KBM - 1) / KBM;
         args.tiles_b = (nb + KBM - 1) / KBM;
         args.ntiles = same ? args.tiles_a * (args.tiles_a + 1) / 2 : args.tiles_a * args.tiles_b;
