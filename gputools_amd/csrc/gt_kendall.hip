@@ -20,9 +20,9 @@
 //   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
 //                      + popc(Ex & Ey)
 // exactly, 6 VALU operations per 32 sample pairs per pair of variables (AND
-// and v_bcnt_u32_b32, which adds the count to an accumulator), or 4 where no
-// variable on one side of the tile has a tie (its E planes are all zero:
-// continuous data).  The count
+// and v_bcnt_u32_b32, which adds the count to an accumulator), or 4 where the
+// E words of the staged chunk are all zero (ties are rare bits in continuous
+// data; a side without any tied variable does not even load them).  The count
 // kernel tiles 128 x 128 pairs of variables over 256 lanes (8 x 8 each),
 // stages word chunks of both sides through LDS and splits the word range over
 // workgroups, adding its counts with 64-bit atomics.  A last pass applies the
@@ -226,9 +226,13 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
             d[2 * KBM] = stage[q].z;
             d[3 * KBM] = stage[q].w;
         }
-        __syncthreads();
+        // units 2 and 5 of every lane are the E words of the two sides: a
+        // chunk where none is set (ties are rare bits) counts two planes
+        const uint4 e0 = stage[2], e1 = stage[5];
+        const bool e_here = (e0.x | e0.y | e0.z | e0.w | e1.x | e1.y | e1.z | e1.w) != 0u;
+        const bool any_e = __syncthreads_or(e_here) != 0;
         if (w + KKW < w_end) gload(w + KKW);
-        if (use_e)
+        if (any_e)
             count_chunk<3>(lds, tx, ty, cnt);
         else
             count_chunk<2>(lds, tx, ty, cnt);
