@@ -14,8 +14,9 @@
 // Here the comparisons are made once per variable.  A planes pass encodes
 // each sample pair (i, j), i < j, of a variable as three bits: P (x_j > x_i),
 // N (x_j < x_i) and E (x_j == x_i); a NaN on either side clears all three.
-// A wave compares x_i with 64 consecutive x_j at once and __ballot turns the
-// comparisons into bit words.  The sample pairs of row i take
+// A wave compares x_i with 64 consecutive x_j at once (four such loads in
+// flight) and __ballot turns the comparisons into bit words, which
+// v_writelane collects across lanes for coalesced 256-byte stores.  The sample pairs of row i take
 // ceil((n - 1 - i) / 32) words (the tail bits are zero), so
 //   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
 //                      + popc(Ex & Ey)
@@ -45,8 +46,14 @@ __host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
     return 16 * q * (q + 1) + r * (q + 1);
 }
 
+// word[lane] = value for a wave-uniform value and lane: one v_writelane_b32.
+__device__ __forceinline__ void put_lane(uint32_t &word, uint32_t value, int lane)
+{
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(word) : "s"(value), "s"(lane));
+}
+
 // ---------------------------------------------------------------------------
-// Planes pass: one wave per (variable, sample i).
+// Planes pass: one workgroup per (variable, row split), one wave per sample i.
 // ---------------------------------------------------------------------------
 // Plane p (0 = P, 1 = N, 2 = E) of variable v, word w at
 // planes[(p * nvar + v) * wpad + w]; row i starts at word
@@ -57,35 +64,75 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
                                                              int64_t nvar, int64_t n, int64_t ldx,
                                                              uint32_t *__restrict__ planes,
                                                              int64_t wpad,
-                                                             uint32_t *__restrict__ ties)
+                                                             uint32_t *__restrict__ ties,
+                                                             int rsplit)
 {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t rows = nvar * n, wtot = words_upto(n - 1);
+    const int64_t wtot = words_upto(n - 1);
     const float qnan = __builtin_nanf("");
-    for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < rows; r += (int64_t)gridDim.x * 4) {
-        const int64_t v = r / n, i = r % n;
+    const int64_t pstride = nvar * wpad;
+    // workgroup (variable v, split rs); its waves take rows
+    // i = 4 rs + wave + 4 rsplit k, a mix of long and short rows
+    const int64_t v = blockIdx.x / rsplit;
+    const int rs = (int)(blockIdx.x % rsplit);
+    const float *xv = x + v * ldx;
+    uint32_t *const pv = planes + v * wpad;
+    bool tied = false;
+    for (int64_t i = 4 * rs + wave; i < n - 1; i += 4 * (int64_t)rsplit) {
         const int64_t m = n - 1 - i;
-        if (m == 0) continue;
-        const float *xv = x + v * ldx;
         const float xi = xv[i];
-        const int64_t base = wtot - words_upto(m);
+        const float *xr = xv + i + 1;                    // x_j, j = i + 1 + q
+        uint32_t *const row = pv + (wtot - words_upto(m));
         const int64_t wr = (m + 31) / 32;
-        // lane l < 6 stores word half (l & 1) of plane (l >> 1)
-        const int plane = lane >> 1, half = lane & 1;
-        uint32_t *dst = planes + ((int64_t)(plane < 3 ? plane : 0) * nvar + v) * wpad + base + half;
-        for (int64_t t = 0; t * 64 < m; ++t) {
-            const int64_t j = i + 1 + t * 64 + lane;
-            const float xj = j < n ? xv[j] : qnan;
-            const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj > xi);
-            const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj < xi);
-            const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj == xi);
-            const unsigned long long mine = plane == 0 ? gt : (plane == 1 ? lt : eq);
-            if (lane < 6 && 2 * t + half < wr) dst[2 * t] = (uint32_t)(mine >> (32 * half));
-            // a tie anywhere in the variable: its E plane is not all zero
-            if (lane == 0 && eq != 0ull) ties[v] = 1u;
+        const int64_t steps = (m + 63) / 64;
+        // lanes 2 (t mod 32) and 2 (t mod 32) + 1 collect the words of step
+        // t; 32 steps (64 words per plane) go out in one store per plane
+        uint32_t wp = 0, wn = 0, we = 0;
+        bool group_tied = false;
+        for (int64_t t0 = 0; t0 < steps; t0 += 4) {
+            // four steps' loads in flight before their comparisons
+            float xj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t q = (t0 + u) * 64 + lane;
+                xj[u] = q < m ? xr[q] : qnan;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t t = t0 + u;
+                if (t >= steps) break;
+                const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
+                const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj[u] < xi);
+                const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
+                const int slot = 2 * (int)(t & 31);
+                put_lane(wp, (uint32_t)gt, slot);
+                put_lane(wp, (uint32_t)(gt >> 32), slot + 1);
+                put_lane(wn, (uint32_t)lt, slot);
+                put_lane(wn, (uint32_t)(lt >> 32), slot + 1);
+                put_lane(we, (uint32_t)eq, slot);
+                put_lane(we, (uint32_t)(eq >> 32), slot + 1);
+                group_tied = group_tied || eq != 0ull;
+                if ((t & 31) == 31 || t + 1 == steps) {
+                    // lanes at or past `used` still hold the previous
+                    // group's words and are not stored; w < wr drops the
+                    // odd last word of the row (it belongs to the next row)
+                    const int used = 2 * (int)(t & 31) + 2;
+                    const int64_t w = 2 * (t & ~int64_t(31)) + lane;
+                    if (lane < used && w < wr) {
+                        row[w] = wp;
+                        row[pstride + w] = wn;
+                        // E words that are zero stay as cleared
+                        if (group_tied) row[2 * pstride + w] = we;
+                    }
+                    tied = tied || group_tied;
+                    group_tied = false;
+                }
+            }
         }
     }
+    // a tie anywhere in the variable: its E plane is not all zero
+    if (lane == 0 && tied) ties[v] = 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -333,18 +380,18 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
         e = launch_fill(reinterpret_cast<float *>(workspace), (int64_t)zero_floats, 0.0f, stream);
         if (e != hipSuccess) return e;
         {
-            const int64_t rows = na * n;
-            int64_t blocks = (rows + 3) / 4;
-            if (blocks > 65536) blocks = 65536;
-            hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
-                               a, na, n, lda, pa, wpad, ta);
+            // row splits per variable: about 16 rows per wave, within the
+            // grid limit (every row is still covered: waves stride over rows)
+            int64_t rs64 = (n + 63) / 64;
+            const int64_t nv = na > nb ? na : nb;
+            if (nv * rs64 > 0x7fffffff) rs64 = 0x7fffffff / nv;
+            const int rsplit = (int)(rs64 > 0 ? rs64 : 1);
+            hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)(na * rsplit)), dim3(256), 0, stream,
+                               a, na, n, lda, pa, wpad, ta, rsplit);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if (!same) {
-                const int64_t rowsb = nb * n;
-                int64_t bb = (rowsb + 3) / 4;
-                if (bb > 65536) bb = 65536;
-                hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)bb), dim3(256), 0, stream,
-                                   b, nb, n, ldb, pb, wpad, tb);
+                hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)(nb * rsplit)), dim3(256), 0,
+                                   stream, b, nb, n, ldb, pb, wpad, tb, rsplit);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         }
