@@ -10,11 +10,16 @@
 // and a pair with d_ij == 0 contributes 0 (subgradient).
 //
 // Row-owner sweep: a workgroup owns 64 rows i and one block of up to 128
-// features, and visits every 64-row block j.  Per (i, j) tile it recomputes
-// d_ij exactly (FP32 differences, two-level summation over 32-feature LDS
-// chunks; D is never stored), forms w_ij = G_ij / d_ij^(p-1) in LDS, then
-// accumulates dX_i over its features.  Every dX row is written by exactly one
-// workgroup: no atomics, deterministic.
+// features, and visits a share of the 64-row blocks j.  Per (i, j) tile it
+// recomputes d_ij exactly (FP32 differences, two-level summation over LDS
+// chunks of up to 32 features; D is never stored), forms
+// w_ij = G_ij / d_ij^(p-1) in LDS, then accumulates dX_i over its features.
+// The 64 x 64 block of packed G is loaded coalesced (lanes along the
+// contiguous index of R's 'dist' order, which is i below the diagonal and j
+// above it) and staged through LDS.
+// When there are few row blocks, the j blocks are split over JS workgroups
+// per row block, which write partial dX into a workspace; a second kernel
+// adds the JS partials in a fixed order.  No atomics: deterministic.
 //
 // The same kernel computes the metric-MDS stress S = sum_{i<j} (d_ij - delta_ij)^2
 // of an embedding Y and its gradient (G_ij = 2 (d_ij - delta_ij)); S is summed
@@ -42,7 +47,8 @@ struct BwArgs {
     const float *g;        // MODE_GRAD: packed dL/dD; MODE_MDS: packed delta
     float *dx;
     int64_t lddx;
-    double *partials;      // MODE_MDS: one per (row block) workgroup
+    float *dxp;            // JS > 1: JS partial dX slices, n x k each
+    double *partials;      // MODE_MDS: one per (row block, j split) workgroup
     float p, pm1;          // p and p - 1
 };
 
@@ -57,13 +63,16 @@ __device__ __forceinline__ float dterm(float t)
 }
 
 // KB: features per workgroup (4, 32 or 128, chosen from k on the host).
+// blockIdx.z = js of JS = gridDim.z: this workgroup visits j blocks
+// js, js + JS, ...; with JS > 1 it writes partial dX to args.dxp.
 template <int OP, int MODE, int KB>
 __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
 {
     constexpr int XLD = KB + 1;   // LDS row length of the x_J feature block
-    __shared__ float xi_s[DKC][DLD];
-    __shared__ float xj_s[DKC][DLD];
-    __shared__ float w_s[BI][WLD];
+    constexpr int DK = KB < DKC ? KB : DKC;   // features per distance chunk
+    __shared__ float xi_s[DK][DLD];
+    __shared__ float xj_s[DK][DLD];
+    __shared__ float w_s[BI][WLD];            // G, then the weights
     __shared__ float xb_s[BJ][XLD];
     __shared__ double red_s[BTH / 64];
 
@@ -89,19 +98,23 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
     }
     double stress = 0.0;
 
-    for (int64_t j0 = 0; j0 < n; j0 += BJ) {
+    // the 64 x 64 block of G is element e = tid + 256 q; below the diagonal
+    // (i > j) the packed index runs along i, above it along j
+    const int js = blockIdx.z, JS = gridDim.z;
+    for (int64_t j0 = (int64_t)js * BJ; j0 < n; j0 += (int64_t)JS * BJ) {
+        const bool upper = j0 > i0;   // every i < j in this tile
         // ---- d_ij for the 64 x 64 tile, exact FP32 differences ------------
         float dsum[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) dsum[a][b] = 0.0f;
-        for (int64_t kb = 0; kb < k; kb += DKC) {
+        for (int64_t kb = 0; kb < k; kb += DK) {
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < (BI * DKC) / BTH; ++q) {
+            for (int q = 0; q < (BI * DK) / BTH; ++q) {
                 const int e = tid + BTH * q;
-                const int row = e >> 5, kc = e & 31;
+                const int row = e / DK, kc = e % DK;
                 const int64_t kk = kb + kc;
                 const int64_t ri = i0 + row, rj = j0 + row;
                 xi_s[kc][row] = (ri < n && kk < k) ? args.x[ri * ldx + kk] : 0.0f;
@@ -114,7 +127,7 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) part[a][b] = 0.0f;
 #pragma unroll 4
-            for (int kc = 0; kc < DKC; ++kc) {
+            for (int kc = 0; kc < DK; ++kc) {
                 const float4 vi = *reinterpret_cast<const float4 *>(&xi_s[kc][ty * 4]);
                 const float4 vj = *reinterpret_cast<const float4 *>(&xj_s[kc][tx * 4]);
                 const float ai[4] = {vi.x, vi.y, vi.z, vi.w};
@@ -135,7 +148,21 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) dsum[a][b] += part[a][b];
         }
-        // ---- weights w_ij = G_ij / d_ij^(p-1) -------------------------------
+        // ---- G to LDS (the previous tile's weights were read before the
+        // distance phase's barriers), then w_ij = G_ij / d_ij^(p-1) ---------
+#pragma unroll
+        for (int q = 0; q < (BI * BJ) / BTH; ++q) {
+            const int e = tid + BTH * q;
+            const int li = upper ? (e >> 6) : (e & 63), lj = upper ? (e & 63) : (e >> 6);
+            const int64_t i = i0 + li, j = j0 + lj;
+            float g = 0.0f;
+            if (i < n && j < n && i != j) {
+                const int64_t hi = i > j ? i : j, lo = i > j ? j : i;
+                g = args.g[packed_base(n, lo) + hi];
+            }
+            w_s[li][lj] = g;
+        }
+        __syncthreads();
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -148,8 +175,7 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
                     const float d = (OP == BW_EUCLID) ? sqrtf(s)
                                     : (OP == BW_MINK3) ? cbrtf(s)
                                                        : powf(s, 1.0f / args.p);
-                    const int64_t hi = i > j ? i : j, lo = i > j ? j : i;
-                    const float gv = args.g[packed_base(n, lo) + hi];
+                    const float gv = w_s[li][lj];
                     if (d > 0.0f) {
                         float gij;
                         if (MODE == MODE_MDS) {
@@ -201,11 +227,14 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
     }
 
     if (gi < n && args.dx) {
+        // JS > 1: partial sums to slice js of the workspace (row stride k)
+        float *const out = JS > 1 ? args.dxp + (int64_t)js * n * k + gi * k
+                                  : args.dx + gi * args.lddx;
 #pragma unroll
         for (int m = 0; m < FPT; ++m) {
             if (asub + 4 * m < fcount) {
                 const float v = (OP == BW_EUCLID) ? fmaf(xi_r[m], rowsum, acc[m]) : acc[m];
-                args.dx[gi * args.lddx + f0 + asub + 4 * m] = v;
+                out[f0 + asub + 4 * m] = v;
             }
         }
     }
@@ -220,7 +249,7 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
         if (tid == 0) {
             double s = 0.0;
             for (int w = 0; w < BTH / 64; ++w) s += red_s[w];
-            args.partials[blockIdx.x] = 0.5 * s;
+            args.partials[(int64_t)js * gridDim.x + blockIdx.x] = 0.5 * s;
         }
     }
 }
@@ -241,10 +270,37 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double *part
     if (threadIdx.x == 0) *out = s_[0];
 }
 
-template <int MODE, int KB>
-hipError_t launch_kb(int op, BwArgs &args, hipStream_t stream)
+// dx = sum over the JS partial slices, in slice order.
+__global__ __launch_bounds__(256) void sum_partial_dx_kernel(const float *__restrict__ dxp, int64_t n,
+                                                             int64_t k, int js_count,
+                                                             float *__restrict__ dx, int64_t lddx)
 {
-    dim3 grid((unsigned)((args.n + BI - 1) / BI), (unsigned)((args.k + KB - 1) / KB));
+    const int64_t total = n * k;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        float s = dxp[e];
+        for (int q = 1; q < js_count; ++q) s += dxp[(int64_t)q * total + e];
+        dx[(e / k) * lddx + e % k] = s;
+    }
+}
+
+int feature_block(int64_t k) { return k <= 4 ? 4 : (k <= 32 ? 32 : KBMAX); }
+
+// j splits: about 2048 workgroups in all, at most 16 splits and no more than
+// the number of j blocks.
+int j_splits(int64_t n, int64_t k)
+{
+    const int64_t gx = (n + BI - 1) / BI, gy = (k + feature_block(k) - 1) / feature_block(k);
+    int64_t js = (2048 + gx * gy - 1) / (gx * gy);
+    if (js > 16) js = 16;
+    if (js > gx) js = gx;
+    return (int)(js < 1 ? 1 : js);
+}
+
+template <int MODE, int KB>
+hipError_t launch_kb(int op, BwArgs &args, int js, hipStream_t stream)
+{
+    dim3 grid((unsigned)((args.n + BI - 1) / BI), (unsigned)((args.k + KB - 1) / KB), (unsigned)js);
     if (op == BW_EUCLID)
         hipLaunchKernelGGL((backward_kernel<BW_EUCLID, MODE, KB>), grid, dim3(BTH), 0, stream, args);
     else if (op == BW_MINK3)
@@ -257,17 +313,36 @@ hipError_t launch_kb(int op, BwArgs &args, hipStream_t stream)
 template <int MODE>
 hipError_t launch_mode(int op, BwArgs &args, hipStream_t stream)
 {
-    if (args.k <= 4) return launch_kb<MODE, 4>(op, args, stream);
-    if (args.k <= 32) return launch_kb<MODE, 32>(op, args, stream);
-    return launch_kb<MODE, KBMAX>(op, args, stream);
+    const int js = j_splits(args.n, args.k);
+    hipError_t e;
+    if (args.k <= 4)
+        e = launch_kb<MODE, 4>(op, args, js, stream);
+    else if (args.k <= 32)
+        e = launch_kb<MODE, 32>(op, args, js, stream);
+    else
+        e = launch_kb<MODE, KBMAX>(op, args, js, stream);
+    if (e != hipSuccess || js == 1 || args.dx == nullptr) return e;
+    int64_t blocks = (args.n * args.k + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(sum_partial_dx_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, args.dxp,
+                       args.n, args.k, js, args.dx, args.lddx);
+    return hipGetLastError();
 }
 
 }  // namespace
 
-int64_t mds_partials_count(int64_t n) { return (n + BI - 1) / BI; }
+int64_t mds_partials_count(int64_t n, int64_t q) { return (n + BI - 1) / BI * j_splits(n, q); }
+
+size_t backward_workspace_bytes(int64_t n, int64_t k)
+{
+    if (n < 2 || k < 1) return 0;
+    const int js = j_splits(n, k);
+    return js > 1 ? (size_t)js * (size_t)n * (size_t)k * sizeof(float) : 0;
+}
 
 hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t ldx, float p,
-                           const float *g_packed, float *dx, int64_t lddx, hipStream_t stream)
+                           const float *g_packed, float *dx, int64_t lddx, float *workspace,
+                           hipStream_t stream)
 {
     if (n <= 0 || k <= 0) return hipSuccess;
     BwArgs args;
@@ -278,6 +353,7 @@ hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t
     args.g = g_packed;
     args.dx = dx;
     args.lddx = lddx;
+    args.dxp = workspace;
     args.partials = nullptr;
     args.p = p;
     args.pm1 = p - 1.0f;
@@ -286,7 +362,7 @@ hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t
 }
 
 hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy, const float *delta,
-                      float *dy, int64_t lddy, double *stress, double *partials,
+                      float *dy, int64_t lddy, double *stress, double *partials, float *workspace,
                       hipStream_t stream)
 {
     BwArgs args;
@@ -297,13 +373,14 @@ hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy, const f
     args.g = delta;
     args.dx = dy;
     args.lddx = lddy;
+    args.dxp = workspace;
     args.partials = partials;
     args.p = 2.0f;
     args.pm1 = 1.0f;
     hipError_t e = launch_mode<MODE_MDS>(BW_EUCLID, args, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, stream, partials,
-                       mds_partials_count(n), stress);
+                       mds_partials_count(n, q), stress);
     return hipGetLastError();
 }
 

@@ -983,8 +983,15 @@ int gt_dist_backward(const float *x, int64_t n, int64_t k, int64_t ldx, int meth
     int op = gt::OP_MINKP;
     if (method == GT_EUCLIDEAN || p == 2.0f) op = gt::OP_EUCLID;
     else if (p == 3.0f) op = gt::OP_MINK3;
-    GT_HIP(gt::launch_backward(op, x, n, k, ldx, p, g_packed, dx, lddx, hs));
-    return GT_OK;
+    // partial dX slices when the row blocks are few (stream-ordered scratch)
+    Scratch ws;
+    int s = ws.acquire(gt::backward_workspace_bytes(n, k), hs);
+    if (s != GT_OK) return s;
+    const hipError_t e = gt::launch_backward(op, x, n, k, ldx, p, g_packed, dx, lddx,
+                                             static_cast<float *>(ws.ptr), hs);
+    const int r = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "gt_dist_backward");
+    return r;
 }
 
 int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy, const float *delta_packed,
@@ -995,9 +1002,11 @@ int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy, const float
     if (stress_dev == nullptr) return fail(GT_ERR_INVALID, "null stress pointer");
     if (n > 1 && (y == nullptr || delta_packed == nullptr)) return fail(GT_ERR_INVALID, "null pointer");
     hipStream_t hs = static_cast<hipStream_t>(stream);
-    const int64_t parts = n > 1 ? gt::mds_partials_count(n) : 1;
+    const int64_t parts = n > 1 ? gt::mds_partials_count(n, q) : 1;
+    const size_t pbytes = round_up((size_t)parts * sizeof(double), 256);
+    const size_t dbytes = (n > 1 && dy != nullptr) ? gt::backward_workspace_bytes(n, q) : 0;
     Scratch ws;
-    int s = ws.acquire(round_up((size_t)parts * sizeof(double), 256), hs);
+    int s = ws.acquire(pbytes + dbytes, hs);
     if (s != GT_OK) return s;
     hipError_t e;
     if (n < 2) {
@@ -1006,7 +1015,10 @@ int gt_mds_stress(const float *y, int64_t n, int64_t q, int64_t ldy, const float
         if (e == hipSuccess && dy != nullptr && n == 1) e = gt::launch_fill(dy, q, 0.0f, hs);
     } else {
         e = gt::launch_mds(y, n, q, ldy, delta_packed, dy, lddy, stress_dev,
-                           static_cast<double *>(ws.ptr), hs);
+                           static_cast<double *>(ws.ptr),
+                           dbytes ? reinterpret_cast<float *>(static_cast<char *>(ws.ptr) + pbytes)
+                                  : nullptr,
+                           hs);
     }
     const int r = ws.release();
     if (e != hipSuccess) return hip_fail(e, "gt_mds_stress");

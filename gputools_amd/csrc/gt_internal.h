@@ -52,14 +52,17 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
                        hipStream_t stream);
 
 // Backward (dL/dX) and fused MDS stress.  op: OP_EUCLID, OP_MINK3 or OP_MINKP.
+// workspace: backward_workspace_bytes(n, k) (partial dX slices when the j
+// blocks are split over workgroups; 0 otherwise).
+size_t backward_workspace_bytes(int64_t n, int64_t k);
 hipError_t launch_backward(int op, const float *x, int64_t n, int64_t k, int64_t ldx,
                            float p, const float *g_packed, float *dx, int64_t lddx,
-                           hipStream_t stream);
-// partials: mds_partials_count(n) doubles of workspace.
-int64_t mds_partials_count(int64_t n);
+                           float *workspace, hipStream_t stream);
+// partials: mds_partials_count(n, q) doubles; workspace as for the backward.
+int64_t mds_partials_count(int64_t n, int64_t q);
 hipError_t launch_mds(const float *y, int64_t n, int64_t q, int64_t ldy,
                       const float *delta, float *dy, int64_t lddy, double *stress,
-                      double *partials, hipStream_t stream);
+                      double *partials, float *workspace, hipStream_t stream);
 
 // Batched persistent kernel.  tile: 1..3 (32, 64, 128 pairs per side).
 int batched_tile_side(int tile);

@@ -46,10 +46,16 @@ __host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
     return 16 * q * (q + 1) + r * (q + 1);
 }
 
-// word[lane] = value for a wave-uniform value and lane: one v_writelane_b32.
+// word[lane] = value for a wave-uniform value and lane: v_writelane_b32 with
+// the lane in M0 (an SGPR lane select beside an SGPR value would exceed the
+// one-scalar-operand limit), after the wait states an M0 write needs before
+// a lane select reads it.
 __device__ __forceinline__ void put_lane(uint32_t &word, uint32_t value, int lane)
 {
-    asm("v_writelane_b32 %0, %1, %2" : "+v"(word) : "s"(value), "s"(lane));
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 3\n\tv_writelane_b32 %0, %1, m0"
+                 : "+v"(word)
+                 : "s"(value), "s"(lane)
+                 : "m0");
 }
 
 // ---------------------------------------------------------------------------
