@@ -20,6 +20,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace gt {
 namespace {
 
@@ -43,7 +45,20 @@ struct PlainArgs {
     float p, inv_p, fk;
 };
 
-template <int OP>
+// Canberra chunk modes.  The quotient |a-b| / |a+b| costs an IEEE divide
+// (about ten instructions) unless the hardware reciprocal can stand in for it:
+// when every staged value of the chunk is 0, non-finite, or of magnitude in
+// [2^-100, 2^100], each nonzero finite |a+b| lies in [2^-123, 2^101] (a sum of
+// such values is a multiple of 2^-123), so v_rcp_f32 (1 ulp) neither takes
+// nor returns a denormal, and |a-b| * rcp(|a+b|) is within 1.5 ulp of the
+// quotient with the same Inf (x/0) and NaN (0/0, Inf/Inf) results.
+//   CB_CLEAN:  no 0 and no non-finite value -- no term can be NaN, so the
+//              per-pair count is the chunk length, added once per chunk;
+//   CB_RANGED: zeros or non-finite values present -- reciprocal + NaN test;
+//   CB_IEEE:   a tiny or huge finite value present -- IEEE divide.
+enum { CB_CLEAN = 0, CB_RANGED = 1, CB_IEEE = 2 };
+
+template <int OP, int MODE = CB_IEEE>
 struct Term {
     // Accumulate one feature of one pair into (part, cnt).
     static __device__ __forceinline__ void add(float &part, int &cnt, float a, float b, float p)
@@ -55,6 +70,13 @@ struct Term {
             part = fmaxf(part, fabsf(a - b));
         } else if (OP == OP_MANHATTAN) {
             part += fabsf(a - b);
+        } else if (OP == OP_CANBERRA && MODE == CB_CLEAN) {
+            part = fmaf(fabsf(a - b), __builtin_amdgcn_rcpf(fabsf(a + b)), part);
+        } else if (OP == OP_CANBERRA && MODE == CB_RANGED) {
+            const float q = fabsf(a - b) * __builtin_amdgcn_rcpf(fabsf(a + b));
+            const bool ok = (q == q);
+            part += ok ? q : 0.0f;
+            cnt += ok ? 1 : 0;
         } else if (OP == OP_CANBERRA) {
             // R's rule: a term with 0 numerator and 0 denominator (a == b == 0,
             // the only case where the quotient is NaN for finite input) is
@@ -132,19 +154,9 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
             cnt[r][c] = 0;
         }
 
-    gload(0);
-    for (int64_t kb = 0; kb < k; kb += PBK) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PLOADS; ++q) {
-            const int e = tid + PTHREADS * q;
-            const int row = e >> 4, kc = e & 15;
-            lds[0][kc][row] = sc[q];
-            lds[1][kc][row] = sr[q];
-        }
-        __syncthreads();
-        if (kb + PBK < k) gload(kb + PBK);
-
+    // One chunk of PBK features from LDS into the micro-tile.
+    auto chunk = [&](auto mode) {
+        constexpr int MODE = decltype(mode)::value;
 #pragma unroll 4
         for (int kc = 0; kc < PBK; ++kc) {
             const float4 c0 = *reinterpret_cast<const float4 *>(&lds[0][kc][tx * 4]);
@@ -156,7 +168,55 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
 #pragma unroll
             for (int r = 0; r < 8; ++r)
 #pragma unroll
-                for (int c = 0; c < 8; ++c) Term<OP>::add(part[r][c], cnt[r][c], cv[c], rv[r], args.p);
+                for (int c = 0; c < 8; ++c)
+                    Term<OP, MODE>::add(part[r][c], cnt[r][c], cv[c], rv[r], args.p);
+        }
+    };
+
+    __shared__ int cb_flags[PTHREADS / 64];  // canberra: per-wave chunk classification
+    int clean_terms = 0;                     // canberra: features of CB_CLEAN chunks
+
+    gload(0);
+    for (int64_t kb = 0; kb < k; kb += PBK) {
+        __syncthreads();
+        bool unclean = false, unranged = false;
+#pragma unroll
+        for (int q = 0; q < PLOADS; ++q) {
+            const int e = tid + PTHREADS * q;
+            const int row = e >> 4, kc = e & 15;
+            lds[0][kc][row] = sc[q];
+            lds[1][kc][row] = sr[q];
+            if (OP == OP_CANBERRA) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const float m = fabsf(s ? sr[q] : sc[q]);
+                    const bool mid = (m >= 0x1p-100f) && (m <= 0x1p100f);
+                    unclean |= !mid;
+                    unranged |= !mid && m != 0.0f && m < __builtin_inff();
+                }
+            }
+        }
+        if (OP == OP_CANBERRA) {
+            const bool wu = __builtin_amdgcn_ballot_w64(unclean) != 0;
+            const bool wr = __builtin_amdgcn_ballot_w64(unranged) != 0;
+            if ((tid & 63) == 0) cb_flags[tid >> 6] = (wu ? 1 : 0) | (wr ? 2 : 0);
+        }
+        __syncthreads();
+        if (kb + PBK < k) gload(kb + PBK);
+
+        if (OP == OP_CANBERRA) {
+            const int4 f = *reinterpret_cast<const int4 *>(cb_flags);
+            const int bits = __builtin_amdgcn_readfirstlane(f.x | f.y | f.z | f.w);
+            if (bits == 0) {
+                chunk(std::integral_constant<int, CB_CLEAN>{});
+                clean_terms += PBK;
+            } else if (bits == 1) {
+                chunk(std::integral_constant<int, CB_RANGED>{});
+            } else {
+                chunk(std::integral_constant<int, CB_IEEE>{});
+            }
+        } else {
+            chunk(std::integral_constant<int, CB_IEEE>{});
         }
         if (OP != OP_MAXIMUM) {
 #pragma unroll
@@ -181,17 +241,17 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
             if (LAYOUT == OUT_PACKED) {
                 if (i > j)
                     args.out[packed_base(args.na, j) + i] =
-                        Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+                        Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk);
             } else if (LAYOUT == OUT_FULL) {
                 if (i > j) {
-                    const float v = Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+                    const float v = Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk);
                     args.out[j * args.ldd + i] = v;
                     args.out[i * args.ldd + j] = v;
                 } else if (i == j) {
                     args.out[i * args.ldd + i] = 0.0f;
                 }
             } else {
-                args.out[j * args.ldd + i] = Term<OP>::finish(total, cnt[r][c], args.inv_p, args.fk);
+                args.out[j * args.ldd + i] = Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk);
             }
         }
     }

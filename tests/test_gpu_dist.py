@@ -149,6 +149,31 @@ def test_canberra_r_edge_rules_on_gpu():
     assert got[orc.packed_index(5, 4, 0)] == math.inf        # a = -b != 0 -> Inf
 
 
+@pytest.mark.parametrize("n,k", [(384, 112), (300, 100)])
+def test_canberra_chunk_modes(n, k):
+    """Each 16-feature chunk takes the reciprocal path when its values allow it
+    (no 0 / non-finite value: clean; zeros or NaN: reciprocal + NaN test) and
+    the IEEE divide otherwise (a tiny or huge magnitude).  One pair's sum mixes
+    all three; 384 rows fill whole 128-row tiles, 300 leaves a zero-padded one."""
+    rng = np.random.default_rng(n + k)
+    x = rng.standard_normal((n, k)).astype(np.float32)
+    x[:, 0:16] = rng.uniform(0.1, 10.0, (n, 16))                      # clean, positive
+    blk = x[:, 32:48]                                                  # zeros and NaN
+    blk[rng.random(blk.shape) < 0.3] = 0.0
+    blk[rng.random(blk.shape) < 0.02] = np.nan
+    x[:, 48:64] *= np.float32(1e-36)                                   # tiny
+    x[:, 52] = rng.standard_normal(n).astype(np.float32) * np.float32(1e-42)  # denormals
+    x[:, 64:80] *= np.float32(1e35)                                    # huge
+    x[:, 80] = np.float32(2.0 ** -100)                                 # the range bounds
+    x[::2, 81] = np.float32(-(2.0 ** 100))
+    x[3, 96:k] = -x[7, 96:k]                                           # a = -b: Inf term
+    print("case canberra chunk modes n=%d k=%d" % (n, k), flush=True)
+    got = run(x, "canberra")
+    want = orc.dist(x, "canberra")
+    assert_parity(got, want, "canberra")
+    assert got[orc.packed_index(n, 7, 3)] == math.inf
+
+
 def test_binary_nonbinary_values_and_zero_rows():
     rng = np.random.default_rng(9)
     x = rng.standard_normal((200, 300)).astype(np.float32)
