@@ -36,7 +36,7 @@ constexpr int BTH = 256;        // threads
 constexpr int DKC = 32;         // features per distance chunk
 constexpr int DLD = BI + 4;     // LDS row length of the k-major distance chunks
 constexpr int KBMAX = 128;      // largest feature block per workgroup
-constexpr int WLD = BJ + 1;     // LDS row length of the weight tile
+constexpr int WLD = BI + 4;     // LDS row length of the G / weight tile (float4 rows)
 
 enum { MODE_GRAD = 0, MODE_MDS = 1 };
 enum { BW_EUCLID = 0, BW_MINK3 = 1, BW_MINKP = 2 };
@@ -68,7 +68,7 @@ __device__ __forceinline__ float dterm(float t)
 template <int OP, int MODE, int KB>
 __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
 {
-    constexpr int XLD = KB + 1;   // LDS row length of the x_J feature block
+    constexpr int XLD = KB + 4;   // LDS row length of the x_J feature block (float4 rows)
     constexpr int DK = KB < DKC ? KB : DKC;   // features per distance chunk
     __shared__ float xi_s[DK][DLD];
     __shared__ float xj_s[DK][DLD];
@@ -84,17 +84,24 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
 
     // distance phase mapping: 4 x 4 pairs per thread
     const int ty = tid >> 4, tx = tid & 15;
-    // accumulation phase mapping: row tid >> 2, features (tid & 3) + 4 m
-    const int arow = tid >> 2, asub = tid & 3;
-    const int64_t gi = i0 + arow;
-    constexpr int FPT = KB / 4;   // features per thread (KB >= 4)
-    float xi_r[FPT], acc[FPT];
-    float rowsum = 0.0f;
+    // accumulation phase mapping (a register-blocked W x_J product): rows
+    // rg * RPT + r, features fg * FPT + m of the feature block
+    constexpr int RPT = KB >= 32 ? 4 : 1;            // rows per thread
+    constexpr int FPT = KB * BI / (BTH * RPT);       // features per thread: 8, 2, 1
+    constexpr int FGROUPS = KB / FPT;
+    const int rg = tid / FGROUPS, fg = tid % FGROUPS;
+    float xi_r[RPT][FPT], acc[RPT][FPT], rowsum[RPT];
 #pragma unroll
-    for (int m = 0; m < FPT; ++m) {
-        const int64_t c = f0 + asub + 4 * m;
-        xi_r[m] = (gi < n && c < k && asub + 4 * m < fcount) ? args.x[gi * ldx + c] : 0.0f;
-        acc[m] = 0.0f;
+    for (int r = 0; r < RPT; ++r) {
+        rowsum[r] = 0.0f;
+        const int64_t gi = i0 + rg * RPT + r;
+#pragma unroll
+        for (int m = 0; m < FPT; ++m) {
+            const int c = fg * FPT + m;
+            // euclidean needs x_i only at the end (dX_i = x_i sum_j w_ij - sum_j w_ij x_j)
+            xi_r[r][m] = (OP != BW_EUCLID && gi < n && c < fcount) ? args.x[gi * ldx + f0 + c] : 0.0f;
+            acc[r][m] = 0.0f;
+        }
     }
     double stress = 0.0;
 
@@ -163,6 +170,7 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
             w_s[li][lj] = g;
         }
         __syncthreads();
+        float w_r[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -195,8 +203,14 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
                         stress += (double)r * (double)r;
                     }
                 }
-                w_s[li][lj] = w;
+                w_r[a][b] = w;
             }
+        // every G value is read: store the weights transposed, w_s[j][i]
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) w_s[tx * 4 + b][ty * 4 + a] = w_r[a][b];
         // ---- x_J feature block ----------------------------------------------
         for (int e = tid; e < BJ * KB; e += BTH) {
             const int row = e / KB, c = e % KB;
@@ -207,34 +221,70 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
         // ---- dX_i += sum_j w_ij phi(x_i - x_j) ------------------------------
         const int jmax = (n - j0) < BJ ? (int)(n - j0) : BJ;
         for (int lj = 0; lj < jmax; ++lj) {
-            const float w = w_s[arow][lj];
-            if (OP == BW_EUCLID) {
-                rowsum += w;
-#pragma unroll
-                for (int m = 0; m < FPT; ++m) acc[m] = fmaf(-w, xb_s[lj][asub + 4 * m], acc[m]);
+            float w[RPT], xj[FPT];
+            if (RPT == 4) {
+                const float4 v = *reinterpret_cast<const float4 *>(&w_s[lj][rg * 4]);
+                w[0] = v.x;
+                w[1] = v.y;
+                w[2 % RPT] = v.z;
+                w[3 % RPT] = v.w;
             } else {
+                w[0] = w_s[lj][rg];
+            }
+            if (FPT % 4 == 0) {
 #pragma unroll
-                for (int m = 0; m < FPT; ++m) {
-                    const float t = xi_r[m] - xb_s[lj][asub + 4 * m];
-                    const float a = fabsf(t);
-                    // sign(t) |t|^(p-1); 0 at t == 0 (p >= 1)
-                    const float ph = (OP == BW_MINK3) ? a * t
-                                     : (a > 0.0f ? copysignf(exp2f(args.pm1 * log2f(a)), t) : 0.0f);
-                    acc[m] = fmaf(w, ph, acc[m]);
+                for (int m = 0; m < FPT; m += 4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(&xb_s[lj][fg * FPT + m]);
+                    xj[m] = v.x;
+                    xj[(m + 1) % FPT] = v.y;
+                    xj[(m + 2) % FPT] = v.z;
+                    xj[(m + 3) % FPT] = v.w;
+                }
+            } else if (FPT == 2) {
+                const float2 v = *reinterpret_cast<const float2 *>(&xb_s[lj][fg * 2]);
+                xj[0] = v.x;
+                xj[1 % FPT] = v.y;
+            } else {
+                xj[0] = xb_s[lj][fg];
+            }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                if (OP == BW_EUCLID) {
+                    rowsum[r] += w[r];
+#pragma unroll
+                    for (int m = 0; m < FPT; ++m) acc[r][m] = fmaf(-w[r], xj[m], acc[r][m]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < FPT; ++m) {
+                        const float t = xi_r[r][m] - xj[m];
+                        const float a = fabsf(t);
+                        // sign(t) |t|^(p-1); 0 at t == 0 (p >= 1)
+                        const float ph = (OP == BW_MINK3) ? a * t
+                                         : (a > 0.0f ? copysignf(exp2f(args.pm1 * log2f(a)), t) : 0.0f);
+                        acc[r][m] = fmaf(w[r], ph, acc[r][m]);
+                    }
                 }
             }
         }
     }
 
-    if (gi < n && args.dx) {
-        // JS > 1: partial sums to slice js of the workspace (row stride k)
-        float *const out = JS > 1 ? args.dxp + (int64_t)js * n * k + gi * k
-                                  : args.dx + gi * args.lddx;
+    if (args.dx) {
 #pragma unroll
-        for (int m = 0; m < FPT; ++m) {
-            if (asub + 4 * m < fcount) {
-                const float v = (OP == BW_EUCLID) ? fmaf(xi_r[m], rowsum, acc[m]) : acc[m];
-                out[f0 + asub + 4 * m] = v;
+        for (int r = 0; r < RPT; ++r) {
+            const int64_t gi = i0 + rg * RPT + r;
+            if (gi >= n) continue;
+            // JS > 1: partial sums to slice js of the workspace (row stride k)
+            float *const out = JS > 1 ? args.dxp + (int64_t)js * n * k + gi * k
+                                      : args.dx + gi * args.lddx;
+#pragma unroll
+            for (int m = 0; m < FPT; ++m) {
+                const int c = fg * FPT + m;
+                if (c < fcount) {
+                    const float v = (OP == BW_EUCLID)
+                                        ? fmaf(args.x[gi * ldx + f0 + c], rowsum[r], acc[r][m])
+                                        : acc[r][m];
+                    out[f0 + c] = v;
+                }
             }
         }
     }
