@@ -21,6 +21,14 @@
 //     when that minimum sat at column sub or sup and the new value does not
 //     settle it; a row sub < r < sup is rescanned only when its minimum sat
 //     at column sub.  Rows whose minimum is unaffected cost one comparison.
+//   - up to n = 9216 the per-row state (counts, cached minima and the new row
+//     sup) stays in LDS for the whole run.  One CU issues every store, and
+//     its scattered column stores (one cache line each) dominate a round, so
+//     this variant writes only row and column sup: a merged-away column is
+//     recognised by its negative count instead of being set to +inf.  Only
+//     the end of a round waits for the stores; until then the rescans take
+//     row sup and column sup from the LDS copy.  A rescanned row is split
+//     over several waves when few rows need one.
 // Ties are broken exactly as the reference's 32-lane reductions do
 // (hcluster.cu:27-57, 67-90): smallest value, then the lane (index mod 32)
 // that wins the reduction tree (bit-reversed lane order), then smallest
@@ -39,9 +47,11 @@ namespace {
 constexpr int HTHREADS = 1024;
 constexpr int HWAVES = HTHREADS / 64;
 constexpr int RESCAN_CAP = 2048;   // rows listed for rescan per round (LDS)
-constexpr int HB = 4;              // columns / rows per thread per batch
+constexpr int HB = 4;              // rows per thread per batch (repair)
+constexpr int HBU = 4;             // columns per thread per batch (update)
 constexpr int NO_COL = -1;         // row without a candidate
 constexpr int PENDING = -2;        // row queued for a rescan
+constexpr int HC_LDS_ROWS = 9216;  // largest n whose row state fits in LDS (16 B per row)
 
 struct HclustArgs {
     float *d;
@@ -124,6 +134,54 @@ __device__ __forceinline__ void scan_row(const float *row, int r, bool vec4, flo
     wave_argmin(bv, bi);
 }
 
+// This lane's argmin of row[c] over c in [c0, c1) (c0 a multiple of 4)
+// among live columns (count[c] > 0: entries of merged-away clusters are not
+// kept at +inf), with column rw taking fixv: the value the current round has
+// stored there, which may still be in flight.
+__device__ __forceinline__ void scan_cols(const float *row, int c0, int c1, int rw, float fixv,
+                                          const float *count, bool vec4, float &bv, int &bi)
+{
+    const int lane = threadIdx.x & 63;
+    auto put = [&](float v, int c) {
+        if (c < c1 && count[c] > 0.0f) take(c == rw ? fixv : v, c, bv, bi);
+    };
+    if (!vec4) {
+        for (int cc = c0 + lane; cc < c1; cc += 64) put(row[cc], cc);
+        return;
+    }
+    // 2048 columns per batch: eight 16 B loads per lane in flight
+    for (int c = c0; c < c1; c += 2048) {
+        float4 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int cc = c + u * 256 + lane * 4;
+            q[u] = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+            if (cc + 3 < c1) {
+                q[u] = *reinterpret_cast<const float4 *>(row + cc);
+            } else if (cc < c1) {
+                q[u].x = row[cc];
+                if (cc + 1 < c1) q[u].y = row[cc + 1];
+                if (cc + 2 < c1) q[u].z = row[cc + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int cc = c + u * 256 + lane * 4;
+            put(q[u].x, cc);
+            put(q[u].y, cc + 1);
+            put(q[u].z, cc + 2);
+            put(q[u].w, cc + 3);
+        }
+    }
+}
+
+// Barrier over LDS traffic only: this wave's global stores may still be in
+// flight when the others pass it.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // A value the compiler cannot see through: a product passed through it is
 // rounded on its own and never fused into the following add (the library is
 // built with -ffp-contract=fast, under which contraction pragmas are ignored).
@@ -202,25 +260,50 @@ __global__ __launch_bounds__(256) void hclust_init_kernel(HclustArgs a)
     }
 }
 
-template <int METHOD>
+// LDSRES: the per-row state (counts, cached minima and the rewritten row
+// sup) lives in LDS for the whole run, n <= HC_LDS_ROWS; otherwise in the
+// global workspace.
+template <int METHOD, bool LDSRES>
 __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
 {
+    extern __shared__ float hs_dyn[];
     __shared__ float red_v[HWAVES];
     __shared__ int red_i[HWAVES];
     __shared__ int s_cw, s_rw, s_nres, s_overflow;
     __shared__ float s_val;
     __shared__ int rescan[RESCAN_CAP];
+    __shared__ float part_v[HWAVES];
+    __shared__ int part_i[HWAVES];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n = a.n;
     const int64_t ldd = a.ldd;
+    float *cntp, *minvp, *newp;
+    int *mincp;
+    if constexpr (LDSRES) {
+        cntp = hs_dyn;
+        minvp = hs_dyn + n;
+        mincp = reinterpret_cast<int *>(hs_dyn + 2 * n);
+        newp = hs_dyn + 3 * n;   // D[sup][c] as rewritten this round
+        for (int r = tid; r < n; r += HTHREADS) {
+            cntp[r] = a.count[r];
+            minvp[r] = a.minv[r];
+            mincp[r] = a.minc[r];
+        }
+        __syncthreads();
+    } else {
+        cntp = a.count;
+        minvp = a.minv;
+        mincp = a.minc;
+        newp = nullptr;
+    }
     for (int iter = 0; iter + 1 < n; ++iter) {
         // ---- the closest pair: argmin over the cached row minima ----------
         float bv = INFINITY;
         int bi = NO_COL;
         for (int r = tid; r < n; r += HTHREADS) {
-            const float v = a.minv[r];
-            if (a.minc[r] >= 0 && v < INFINITY && (bi == NO_COL || beats(v, r, bv, bi))) {
+            const float v = minvp[r];
+            if (mincp[r] >= 0 && v < INFINITY && (bi == NO_COL || beats(v, r, bv, bi))) {
                 bv = v;
                 bi = r;
             }
@@ -230,7 +313,10 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             red_v[wave] = bv;
             red_i[wave] = bi;
         }
-        __syncthreads();
+        if constexpr (LDSRES)
+            lds_barrier();
+        else
+            __syncthreads();
         if (tid == 0) {
             float v = INFINITY;
             int r = NO_COL;
@@ -242,13 +328,13 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             int cw, rw;
             if (r != NO_COL) {
                 rw = r;
-                cw = a.minc[r];
+                cw = mincp[r];
             } else {
                 // no finite distance left: the two lowest-index clusters
                 cw = -1;
                 rw = -1;
                 for (int i = 0; i < n && rw < 0; ++i)
-                    if (a.count[i] > 0.0f) {
+                    if (cntp[i] > 0.0f) {
                         if (cw < 0)
                             cw = i;
                         else
@@ -259,51 +345,70 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             a.height[iter] = v;
             a.sub[iter] = cw;
             a.sup[iter] = rw;
-            a.count[rw] += a.count[cw];
-            a.count[cw] *= -1.0f;
+            cntp[rw] += cntp[cw];
+            cntp[cw] *= -1.0f;
             s_cw = cw;
             s_rw = rw;
             s_val = v;
             s_nres = 0;
             s_overflow = 0;
         }
-        __syncthreads();
+        if constexpr (LDSRES)
+            lds_barrier();
+        else
+            __syncthreads();
         const int cw = s_cw, rw = s_rw;
         const float val = s_val;
-        const float nr = a.count[rw];
-        const float np = -a.count[cw];
+        const float nr = cntp[rw];
+        const float np = -cntp[cw];
         const float nq = nr - np;
 
         // ---- rewrite row/column sup; row/column sub become +inf ----------
         float *const dcw = a.d + (int64_t)cw * ldd;
         float *const drw = a.d + (int64_t)rw * ldd;
-        // batches of HB columns per thread: every load of a batch is issued
+        // batches of HBU columns per thread: every load of a batch is issued
         // before its stores, which could otherwise alias them
-        for (int c0 = tid; c0 < n; c0 += HTHREADS * HB) {
-            float bot[HB], top[HB], nk[HB];
+        // (8 columns per thread in the LDS variant: one batch of loads at
+        // n = 8192, none queued behind the first batch's column stores)
+        constexpr int HBC = LDSRES ? 2 * HBU : HBU;
+        for (int c0 = tid; c0 < n; c0 += HTHREADS * HBC) {
+            float bot[HBC], top[HBC], nk[HBC];
 #pragma unroll
-            for (int u = 0; u < HB; ++u) {
+            for (int u = 0; u < HBC; ++u) {
                 const int c = c0 + u * HTHREADS;
                 if (c < n && c != cw && c != rw) {
                     bot[u] = drw[c];
                     top[u] = dcw[c];
-                    nk[u] = a.count[c];
+                    if constexpr (!LDSRES) nk[u] = cntp[c];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < HB; ++u) {
+            for (int u = 0; u < HBC; ++u) {
                 const int c = c0 + u * HTHREADS;
                 if (c >= n) break;
                 float v = INFINITY;
-                if (c != cw && c != rw)
-                    v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nk[u], val, a.lambda, a.beta);
-                dcw[c] = INFINITY;
-                a.d[(int64_t)c * ldd + cw] = INFINITY;
+                if constexpr (LDSRES) {
+                    // row and column sub are left as they are (sub is dead,
+                    // and scans skip dead columns); a dead column c holds a
+                    // stale value, so its new entry is +inf
+                    const float nkc = cntp[c];
+                    if (c != cw && c != rw && nkc > 0.0f)
+                        v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nkc, val, a.lambda, a.beta);
+                } else {
+                    if (c != cw && c != rw)
+                        v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nk[u], val, a.lambda, a.beta);
+                    dcw[c] = INFINITY;
+                    a.d[(int64_t)c * ldd + cw] = INFINITY;
+                }
                 drw[c] = v;
                 a.d[(int64_t)c * ldd + rw] = v;
+                if constexpr (LDSRES) newp[c] = v;
             }
         }
-        __syncthreads();
+        if constexpr (LDSRES)
+            lds_barrier();
+        else
+            __syncthreads();
 
         // ---- repair the cached row minima --------------------------------
         for (int r0 = cw + tid; r0 < n; r0 += HTHREADS * HB) {
@@ -313,10 +418,10 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             for (int u = 0; u < HB; ++u) {
                 const int r = r0 + u * HTHREADS;
                 if (r < n) {
-                    cnt[u] = a.count[r];
-                    mvs[u] = a.minv[r];
-                    mcs[u] = a.minc[r];
-                    fs[u] = drw[r];
+                    cnt[u] = cntp[r];
+                    mvs[u] = minvp[r];
+                    mcs[u] = mincp[r];
+                    fs[u] = LDSRES ? newp[r] : drw[r];
                 }
             }
 #pragma unroll
@@ -325,8 +430,8 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             if (r >= n) break;
             bool redo = false;
             if (r == cw) {
-                a.minv[r] = INFINITY;
-                a.minc[r] = NO_COL;
+                minvp[r] = INFINITY;
+                mincp[r] = NO_COL;
                 continue;
             }
             if (cnt[u] <= 0.0f) continue;
@@ -341,19 +446,19 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
                 if (mc == cw) {
                     redo = !(f < mv);
                     if (!redo) {
-                        a.minv[r] = f;
-                        a.minc[r] = rw;
+                        minvp[r] = f;
+                        mincp[r] = rw;
                     }
                 } else if (mc == rw) {
                     redo = !(f <= mv);
-                    if (!redo) a.minv[r] = f;
+                    if (!redo) minvp[r] = f;
                 } else if (f < INFINITY && (mc == NO_COL || beats(f, rw, mv, mc))) {
-                    a.minv[r] = f;
-                    a.minc[r] = rw;
+                    minvp[r] = f;
+                    mincp[r] = rw;
                 }
             }
             if (redo) {
-                a.minc[r] = PENDING;
+                mincp[r] = PENDING;
                 const int slot = atomicAdd(&s_nres, 1);
                 if (slot < RESCAN_CAP)
                     rescan[slot] = r;
@@ -362,30 +467,116 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             }
             }
         }
-        __syncthreads();
+        if constexpr (LDSRES)
+            lds_barrier();
+        else
+            __syncthreads();
 
-        // ---- rescans: one wave per row -----------------------------------
+        // ---- rescans ------------------------------------------------------
         const int nres = s_nres < RESCAN_CAP ? s_nres : RESCAN_CAP;
+        if constexpr (LDSRES) {
+            // This round's stores to row and column sup may still be in
+            // flight: row sup is read from its LDS copy, and in other rows
+            // column sup takes the LDS copy's value.  Dead columns (sub's
+            // included) are skipped by their count.
+            const float *const dr = a.d;
+            if (nres < HWAVES) {
+                // few rows: each is split over HWAVES / nres waves, whose
+                // partial argmins are combined (the order of (value, tie key)
+                // is total, so any grouping gives the same minimum)
+                const int S = nres > 0 ? HWAVES / nres : 1;
+                const int q = wave / S, sg = wave % S;
+                float v = INFINITY;
+                int c = NO_COL;
+                if (q < nres) {
+                    const int r = rescan[q];
+                    const int per = (((r + 2047) >> 11) + S - 1) / S;   // 2048-column batches
+                    const int c0 = min(r, sg * per * 2048), c1 = min(r, (sg + 1) * per * 2048);
+                    if (r == rw) {
+                        for (int cc = c0 + lane; cc < c1; cc += 64) take(newp[cc], cc, v, c);
+                    } else {
+                        scan_cols(dr + (int64_t)r * ldd, c0, c1, rw, newp[r], cntp, a.vec4, v, c);
+                    }
+                }
+                wave_argmin(v, c);
+                if (lane == 0) {
+                    part_v[wave] = v;
+                    part_i[wave] = c;
+                }
+                lds_barrier();
+                if (tid < nres) {
+                    float bv = INFINITY;
+                    int bi = NO_COL;
+                    for (int w = tid * S; w < tid * S + S; ++w)
+                        if (part_i[w] != NO_COL && (bi == NO_COL || beats(part_v[w], part_i[w], bv, bi))) {
+                            bv = part_v[w];
+                            bi = part_i[w];
+                        }
+                    minvp[rescan[tid]] = bv;
+                    mincp[rescan[tid]] = bi;
+                }
+            } else {
+                for (int q = wave; q < nres; q += HWAVES) {
+                    const int r = rescan[q];
+                    float v = INFINITY;
+                    int c = NO_COL;
+                    if (r == rw) {
+                        for (int cc = lane; cc < r; cc += 64) take(newp[cc], cc, v, c);
+                    } else {
+                        scan_cols(dr + (int64_t)r * ldd, 0, r, rw, newp[r], cntp, a.vec4, v, c);
+                    }
+                    wave_argmin(v, c);
+                    if (lane == 0) {
+                        minvp[r] = v;
+                        mincp[r] = c;
+                    }
+                }
+                if (s_overflow) {
+                    // more rows than the list holds: find the rest by their
+                    // marker (every listed row has been rescanned by now)
+                    lds_barrier();
+                    for (int r = cw + 1 + wave; r < n; r += HWAVES) {
+                        if (mincp[r] != PENDING) continue;
+                        float v = INFINITY;
+                        int c = NO_COL;
+                        if (r == rw) {
+                            for (int cc = lane; cc < r; cc += 64) take(newp[cc], cc, v, c);
+                        } else {
+                            scan_cols(dr + (int64_t)r * ldd, 0, r, rw, newp[r], cntp, a.vec4, v, c);
+                        }
+                        wave_argmin(v, c);
+                        if (lane == 0) {
+                            minvp[r] = v;
+                            mincp[r] = c;
+                        }
+                    }
+                }
+            }
+            // this round's global stores complete before the next round
+            // reads rows sup and sub
+            __syncthreads();
+            continue;
+        }
         for (int q = wave; q < nres; q += HWAVES) {
             const int r = rescan[q];
             float v;
             int c;
             scan_row(a.d + (int64_t)r * ldd, r, a.vec4, v, c);
             if (lane == 0) {
-                a.minv[r] = v;
-                a.minc[r] = c;
+                minvp[r] = v;
+                mincp[r] = c;
             }
         }
         if (s_overflow) {
             // more rows than the list holds: find the rest by their marker
             for (int r = cw + 1 + wave; r < n; r += HWAVES) {
-                if (a.minc[r] != PENDING) continue;
+                if (mincp[r] != PENDING) continue;
                 float v;
                 int c;
                 scan_row(a.d + (int64_t)r * ldd, r, a.vec4, v, c);
                 if (lane == 0) {
-                    a.minv[r] = v;
-                    a.minc[r] = c;
+                    minvp[r] = v;
+                    mincp[r] = c;
                 }
             }
         }
@@ -419,10 +610,23 @@ hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lam
     hipLaunchKernelGGL(hclust_init_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    // 16 B of LDS per row (count, cached minimum and its column, the new row
+    // sup) next to the 8 KB of static LDS: 155 KB at the largest n
+    const bool lds = n <= HC_LDS_ROWS;
+    const unsigned shm = lds ? (unsigned)(16 * n) : 0u;
     switch (method) {
 #define GT_HC_CASE(m)                                                                          \
     case m:                                                                                    \
-        hipLaunchKernelGGL(hclust_merge_kernel<m>, dim3(1), dim3(HTHREADS), 0, stream, a);     \
+        if (lds) {                                                                             \
+            e = hipFuncSetAttribute(reinterpret_cast<const void *>(hclust_merge_kernel<m, true>), \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);     \
+            if (e != hipSuccess) return e;                                                     \
+            hipLaunchKernelGGL((hclust_merge_kernel<m, true>), dim3(1), dim3(HTHREADS), shm,   \
+                               stream, a);                                                     \
+        } else {                                                                               \
+            hipLaunchKernelGGL((hclust_merge_kernel<m, false>), dim3(1), dim3(HTHREADS), 0,    \
+                               stream, a);                                                     \
+        }                                                                                      \
         break;
     GT_HC_CASE(HC_SINGLE)
     GT_HC_CASE(HC_COMPLETE)

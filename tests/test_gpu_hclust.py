@@ -115,3 +115,38 @@ def test_gpudistclust_single_point_and_pair():
     assert t1.merge.shape == (0, 2) and list(t1.order) == [1]
     t2 = gt.gpuDistClust(np.array([[0.0, 0.0], [3.0, 4.0]]))
     assert t2.merge.tolist() == [[-1, -2]] and t2.height.tolist() == [5.0]
+
+
+def mst_weights(d):
+    """Sorted edge weights of a minimum spanning tree of the complete graph
+    on the symmetric matrix d (Prim, O(n^2)).  Single linkage merges at
+    exactly these heights, whatever the order of equal values."""
+    n = d.shape[0]
+    in_tree = np.zeros(n, bool)
+    in_tree[0] = True
+    best = d[0].copy()
+    w = np.empty(n - 1, np.float32)
+    for t in range(n - 1):
+        cand = np.where(in_tree, np.float32(np.inf), best)
+        j = int(np.argmin(cand))
+        w[t] = cand[j]
+        in_tree[j] = True
+        np.minimum(best, d[j], out=best)
+    return np.sort(w)
+
+
+@pytest.mark.parametrize("n", [9216, 9300])
+def test_single_linkage_heights_are_the_mst_weights(n):
+    """Sizes the reference-order oracle cannot reach: the per-row state is
+    held in LDS up to n = 9216 and in global memory above; both must give
+    single linkage's closed form (the sorted minimum-spanning-tree weights)."""
+    rng = np.random.default_rng(n)
+    a = rng.random((n, n), dtype=np.float32)
+    d = (a + a.T) * np.float32(0.5)
+    del a
+    print("case hclust single linkage n=%d" % n, flush=True)
+    want = mst_weights(d)
+    sub, sup, h = run_gpu(d, "single")
+    assert (sub < sup).all()
+    np.testing.assert_array_equal(np.sort(h), want)
+    assert (np.diff(h) >= 0).all()          # single linkage heights never decrease

@@ -155,3 +155,30 @@ def test_gpuhclust_argument_checks():
         gt.gpuHclust(gt.Dist(np.zeros(0), 1))
     with pytest.raises(ValueError, match="improper length"):
         gt.gpuHclust(gt.Dist(np.zeros(2), 3))
+
+
+def test_oracle_single_linkage_heights_are_mst_weights():
+    """Single linkage merges at the minimum-spanning-tree edge weights
+    (Kruskal over the pairs, union-find), equal values included."""
+    rng = np.random.default_rng(31)
+    for n, levels in ((40, None), (120, 7)):
+        a = rng.random((n, n), dtype=np.float32)
+        if levels:
+            a = np.floor(a * levels).astype(np.float32)   # many equal distances
+        d = (a + a.T) * np.float32(0.5)
+        iu = np.triu_indices(n, 1)
+        parent = list(range(n))
+
+        def root(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+        mst = []
+        for v, i, j in sorted(zip(d[iu], iu[0], iu[1])):
+            ri, rj = root(i), root(j)
+            if ri != rj:
+                parent[ri] = rj
+                mst.append(v)
+        _, _, h = orc.hclust(d.copy(), "single")
+        np.testing.assert_array_equal(np.sort(h), np.array(mst, np.float32))
