@@ -36,6 +36,8 @@
 #include "gt_common.h"
 #include "gt_internal.h"
 
+#include <type_traits>
+
 namespace gt {
 namespace {
 
@@ -46,16 +48,37 @@ __host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
     return 16 * q * (q + 1) + r * (q + 1);
 }
 
-// word[lane] = value for a wave-uniform value and lane: v_writelane_b32 with
-// the lane in M0 (an SGPR lane select beside an SGPR value would exceed the
-// one-scalar-operand limit), after the wait states an M0 write needs before
-// a lane select reads it.
-__device__ __forceinline__ void put_lane(uint32_t &word, uint32_t value, int lane)
+// The six words of one step into lanes L and L + 1 of the plane words:
+// v_writelane_b32 with the lane select as an inline constant.  The values
+// are compare results just written to SGPRs by the VALU; the compiler does
+// not see inside the asm, so the s_nop supplies the wait states between
+// that write and the lane writes that read it.
+template <int L>
+__device__ __forceinline__ void put_step(uint32_t &wp, uint32_t &wn, uint32_t &we,
+                                         unsigned long long gt, unsigned long long lt,
+                                         unsigned long long eq)
 {
-    asm volatile("s_mov_b32 m0, %2\n\ts_nop 3\n\tv_writelane_b32 %0, %1, m0"
-                 : "+v"(word)
-                 : "s"(value), "s"(lane)
-                 : "m0");
+    asm volatile("s_nop 4\n\t"
+                 "v_writelane_b32 %0, %3, %9\n\t"
+                 "v_writelane_b32 %0, %4, %10\n\t"
+                 "v_writelane_b32 %1, %5, %9\n\t"
+                 "v_writelane_b32 %1, %6, %10\n\t"
+                 "v_writelane_b32 %2, %7, %9\n\t"
+                 "v_writelane_b32 %2, %8, %10"
+                 : "+v"(wp), "+v"(wn), "+v"(we)
+                 : "s"((uint32_t)gt), "s"((uint32_t)(gt >> 32)), "s"((uint32_t)lt),
+                   "s"((uint32_t)(lt >> 32)), "s"((uint32_t)eq), "s"((uint32_t)(eq >> 32)),
+                   "i"(L), "i"(L + 1));
+}
+
+// f(std::integral_constant<int, U>) for U = 0 .. N - 1, in order.
+template <int U, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (U < N) {
+        f(std::integral_constant<int, U>{});
+        static_for<U + 1, N>(f);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -92,49 +115,41 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
         uint32_t *const row = pv + (wtot - words_upto(m));
         const int64_t wr = (m + 31) / 32;
         const int64_t steps = (m + 63) / 64;
-        // lanes 2 (t mod 32) and 2 (t mod 32) + 1 collect the words of step
-        // t; 32 steps (64 words per plane) go out in one store per plane
-        uint32_t wp = 0, wn = 0, we = 0;
-        bool group_tied = false;
-        for (int64_t t0 = 0; t0 < steps; t0 += 4) {
-            // four steps' loads in flight before their comparisons
-            float xj[4];
+        // groups of 32 steps: lanes 2u and 2u + 1 collect the words of step
+        // u (a compile-time lane, so each word is one v_writelane), and the
+        // group's 64 words per plane go out in one store per plane
+        for (int64_t g0 = 0; g0 < steps; g0 += 32) {
+            const int nst = (int)((steps - g0) < 32 ? (steps - g0) : 32);
+            float xj[32];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int64_t q = (t0 + u) * 64 + lane;
+            for (int u = 0; u < 32; ++u) {
+                if (u >= nst) break;
+                const int64_t q = (g0 + u) * 64 + lane;
                 xj[u] = q < m ? xr[q] : qnan;
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int64_t t = t0 + u;
-                if (t >= steps) break;
-                const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
-                const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj[u] < xi);
-                const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
-                const int slot = 2 * (int)(t & 31);
-                put_lane(wp, (uint32_t)gt, slot);
-                put_lane(wp, (uint32_t)(gt >> 32), slot + 1);
-                put_lane(wn, (uint32_t)lt, slot);
-                put_lane(wn, (uint32_t)(lt >> 32), slot + 1);
-                put_lane(we, (uint32_t)eq, slot);
-                put_lane(we, (uint32_t)(eq >> 32), slot + 1);
-                group_tied = group_tied || eq != 0ull;
-                if ((t & 31) == 31 || t + 1 == steps) {
-                    // lanes at or past `used` still hold the previous
-                    // group's words and are not stored; w < wr drops the
-                    // odd last word of the row (it belongs to the next row)
-                    const int used = 2 * (int)(t & 31) + 2;
-                    const int64_t w = 2 * (t & ~int64_t(31)) + lane;
-                    if (lane < used && w < wr) {
-                        row[w] = wp;
-                        row[pstride + w] = wn;
-                        // E words that are zero stay as cleared
-                        if (group_tied) row[2 * pstride + w] = we;
-                    }
-                    tied = tied || group_tied;
-                    group_tied = false;
+            uint32_t wp = 0, wn = 0, we = 0;
+            bool group_tied = false;
+            static_for<0, 32>([&](auto uc) {
+                constexpr int u = decltype(uc)::value;
+                if (u < nst) {
+                    const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
+                    const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj[u] < xi);
+                    const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
+                    put_step<2 * u>(wp, wn, we, gt, lt, eq);
+                    group_tied = group_tied || eq != 0ull;
                 }
+            });
+            // lanes at or past `used` hold no word of this group; w < wr
+            // drops the odd last word of the row (it belongs to the next row)
+            const int used = 2 * nst;
+            const int64_t w = 2 * g0 + lane;
+            if (lane < used && w < wr) {
+                row[w] = wp;
+                row[pstride + w] = wn;
+                // E words that are zero stay as cleared
+                if (group_tied) row[2 * pstride + w] = we;
             }
+            tied = tied || group_tied;
         }
     }
     // a tie anywhere in the variable: its E plane is not all zero
