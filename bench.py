@@ -259,16 +259,17 @@ def path_pmcc(torch, gt, wl, steps, warmup, out):
 
 def path_kendall(torch, gt, wl, steps, warmup, out):
     # gpuCor(x, method = "kendall"): 1024 variables x 2048 samples N(0,1),
-    # self case.  Algorithmic work of the count: per unique pair of variables
-    # and 32 sample pairs, 3 ANDs and 3 popcount-accumulates (32-bit integer
-    # VALU lane operations; 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s).
+    # self case.  Algorithmic work of the count for variables without ties or
+    # NaN (this workload): per unique pair of variables and 32 sample pairs,
+    # one XOR and one popcount-accumulate (32-bit integer VALU lane
+    # operations; 256 CUs x 64 lanes x 2.4 GHz = 39.3 T/s).
     nv, size = 1024, 2048
     a = torch.from_numpy(wl.normal(nv, size, seed=8)).cuda()
     tau = torch.empty((nv, nv), dtype=torch.float64, device="cuda")
     ms = _timed(lambda: gt.kendall(a, out=tau), steps, warmup, torch)
     q, r = divmod(size - 1, 32)
     words = 16 * q * (q + 1) + r * (q + 1)
-    ops = 6.0 * words * nv * (nv + 1) / 2
+    ops = 2.0 * words * nv * (nv + 1) / 2
     tops = ops / (ms * 1e-3) / 1e12
     out["kendall_self_1024x2048"] = {
         "ms": round(ms, 4), "pairs_per_s": round(nv * nv / (ms * 1e-3), 1),

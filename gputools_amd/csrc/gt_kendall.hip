@@ -23,7 +23,12 @@
 // exactly, 6 VALU operations per 32 sample pairs per pair of variables (AND
 // and v_bcnt_u32_b32, which adds the count to an accumulator), or 4 where the
 // E words of the staged chunk are all zero (ties are rare bits in continuous
-// data; a side without any tied variable does not even load them).  The count
+// data; a side without any tied variable does not even load them).  In a
+// tile without NaN, a chunk without E bits has exactly one of the P and N
+// bits of every sample pair set, so
+//   concordant(x, y) = sum over words of popc(~(Px ^ Py)) - (bits past row ends),
+// 2 operations per 32 sample pairs (the bits past row ends are counted once
+// per chunk from the first variable's P | N | E).  The count
 // kernel tiles 128 x 128 pairs of variables over 256 lanes (8 x 8 each),
 // stages word chunks of both sides through LDS and splits the word range over
 // workgroups, adding its counts with 64-bit atomics.  A last pass applies the
@@ -87,8 +92,10 @@ __device__ __forceinline__ void static_for(F &&f)
 // Plane p (0 = P, 1 = N, 2 = E) of variable v, word w at
 // planes[(p * nvar + v) * wpad + w]; row i starts at word
 // words_upto(n - 1) - words_upto(n - 1 - i).  Words past the last row are
-// zero (cleared before this pass).  ties[v] (cleared before) becomes 1 when
-// any sample pair of v is tied, i.e. its E plane has a set bit.
+// zero (cleared before this pass).  flags[v] (cleared before) gets bit 0 when
+// any sample pair of v is tied (its E plane has a set bit) and bit 1 when v
+// holds a NaN (every x_j with j < n - 1 is the x_i of a row; x_{n-1} is
+// checked on its own).
 __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__restrict__ x,
                                                              int64_t nvar, int64_t n, int64_t ldx,
                                                              uint32_t *__restrict__ planes,
@@ -107,10 +114,11 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
     const int rs = (int)(blockIdx.x % rsplit);
     const float *xv = x + v * ldx;
     uint32_t *const pv = planes + v * wpad;
-    bool tied = false;
+    bool tied = false, has_nan = false;
     for (int64_t i = 4 * rs + wave; i < n - 1; i += 4 * (int64_t)rsplit) {
         const int64_t m = n - 1 - i;
         const float xi = xv[i];
+        has_nan = has_nan || xi != xi;
         const float *xr = xv + i + 1;                    // x_j, j = i + 1 + q
         uint32_t *const row = pv + (wtot - words_upto(m));
         const int64_t wr = (m + 31) / 32;
@@ -153,7 +161,9 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
         }
     }
     // a tie anywhere in the variable: its E plane is not all zero
-    if (lane == 0 && tied) ties[v] = 1u;
+    if (rs == 0 && wave == 0 && n >= 1 && xv[n - 1] != xv[n - 1]) has_nan = true;
+    const uint32_t bits = (tied ? 1u : 0u) | (has_nan ? 2u : 0u);
+    if (lane == 0 && bits) atomicOr(ties + v, bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,7 +181,7 @@ struct KendallArgs {
     int64_t ldc;
     int64_t tiles_a, tiles_b, ntiles;
     int64_t split_w;           // words per workgroup (multiple of KKW)
-    const uint32_t *ties_a, *ties_b;   // per variable: 1 if its E plane has a set bit
+    const uint32_t *ties_a, *ties_b;   // per variable: bit 0 a tie (E bit set), bit 1 a NaN
     int64_t id_offset, ids_this;
 };
 
@@ -211,6 +221,33 @@ __device__ __forceinline__ void count_chunk(const uint32_t (&lds)[2][3][KKW][KBM
     }
 }
 
+// Chunks without E bits in tiles without NaN: every sample pair has exactly
+// one of its P and N bits, so the pair is concordant where the P bits agree:
+// one XNOR and one v_bcnt_u32_b32 per word and pair.  The bits past the end
+// of a row agree too (all zero); the caller subtracts them per chunk.
+__device__ __forceinline__ void count_chunk_xnor(const uint32_t (&lds)[2][3][KKW][KBM], int tx,
+                                                 int ty, uint32_t (&cnt)[8][8])
+{
+#pragma unroll 2
+    for (int kw = 0; kw < KKW; ++kw) {
+        const uint4 a0 = *reinterpret_cast<const uint4 *>(&lds[0][0][kw][ty * 4]);
+        const uint4 a1 = *reinterpret_cast<const uint4 *>(&lds[0][0][kw][64 + ty * 4]);
+        const uint4 b0 = *reinterpret_cast<const uint4 *>(&lds[1][0][kw][tx * 4]);
+        const uint4 b1 = *reinterpret_cast<const uint4 *>(&lds[1][0][kw][64 + tx * 4]);
+        const uint32_t xa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+        const uint32_t yb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                uint32_t s = cnt[r][c];
+                s += __builtin_popcount(~(xa[r] ^ yb[c]));
+                asm("" : "+v"(s));
+                cnt[r][c] = s;
+            }
+    }
+}
+
 // SYM: pa == pb; tiles with J <= I only (the finish pass reads the other half
 // from the transposed position).
 template <bool SYM>
@@ -237,21 +274,24 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
     if (w_end > args.wpad) w_end = args.wpad;
     if (w_begin >= w_end) return;
 
-    // The E planes only matter when a variable on each side has a tie; for
+    // Flags of the tile's variables (A in lanes 0..127, B in 128..255).  The
+    // E planes only matter when a variable on each side has a tie; for
     // tie-free tiles (continuous data) they are all zero and are skipped:
-    // two planes, 4 instead of 6 operations per word.
-    int tie = 0;
+    // two planes, 4 instead of 6 operations per word.  A tile with no tie and
+    // no NaN at all takes the XOR form below.
+    uint32_t f = 0;
     if (tid < KBM) {
         const int64_t v = x0 + tid;
-        tie = v < args.na ? (int)args.ties_a[v] : 0;
+        f = v < args.na ? args.ties_a[v] : 0u;
+    } else {
+        const int64_t v = y0 + (tid - KBM);
+        f = v < args.nb ? args.ties_b[v] : 0u;
     }
-    const bool tie_x = __syncthreads_or(tie) != 0;
-    tie = 0;
-    if (tid < KBM) {
-        const int64_t v = y0 + tid;
-        tie = v < args.nb ? (int)args.ties_b[v] : 0;
-    }
-    const bool use_e = tie_x && (__syncthreads_or(tie) != 0);
+    const bool tie_x = __syncthreads_or(tid < KBM && (f & 1u)) != 0;
+    const bool tie_y = __syncthreads_or(tid >= KBM && (f & 1u)) != 0;
+    const bool use_e = tie_x && tie_y;
+    // any NaN in the tile: the XNOR form below is not used
+    const bool nan_tile = __syncthreads_or((f & 2u) != 0u) != 0;
 
     // staging unit u = tid + 256 q: half u & 1 (words 4 half .. +3 of the
     // chunk), variable (u >> 1) & 127, side/plane u >> 8 (side = sp / 3)
@@ -264,7 +304,9 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const int side = sp / 3, plane = sp % 3;
         const int64_t nside = side ? args.nb : args.na;
         const int64_t gv = (side ? y0 : x0) + v;
-        ok[q] = gv < nside && (plane < 2 || use_e);
+        // E words are loaded for a side with a tied variable: E & E needs
+        // both sides, the XNOR form's check needs either
+        ok[q] = gv < nside && (plane < 2 || (side ? tie_y : tie_x));
         src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (ok[q] ? gv : 0)) * args.wpad +
                  half * 4;
     }
@@ -281,6 +323,7 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
 #pragma unroll
         for (int c = 0; c < 8; ++c) cnt[r][c] = 0;
 
+    uint32_t past_end = 0;   // XNOR chunks: bits past row ends, counted for every pair
     gload(w_begin);
     for (int64_t w = w_begin; w < w_end; w += KKW) {
         __syncthreads();
@@ -300,10 +343,19 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const bool e_here = (e0.x | e0.y | e0.z | e0.w | e1.x | e1.y | e1.z | e1.w) != 0u;
         const bool any_e = __syncthreads_or(e_here) != 0;
         if (w + KKW < w_end) gload(w + KKW);
-        if (any_e)
+        if (!any_e && !nan_tile) {
+            // the chunk's bits past row ends, from the first variable (with
+            // no NaN, P | N | E marks every sample pair)
+#pragma unroll
+            for (int kw = 0; kw < KKW; ++kw)
+                past_end += 32u - (uint32_t)__builtin_popcount(lds[0][0][kw][0] | lds[0][1][kw][0] |
+                                                               lds[0][2][kw][0]);
+            count_chunk_xnor(lds, tx, ty, cnt);
+        } else if (any_e && use_e) {
             count_chunk<3>(lds, tx, ty, cnt);
-        else
+        } else {
             count_chunk<2>(lds, tx, ty, cnt);
+        }
     }
 
 #pragma unroll
@@ -314,7 +366,7 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         for (int c = 0; c < 8; ++c) {
             const int64_t y = y0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
             if (y >= args.nb) continue;
-            atomicAdd(args.count + x * args.ldc + y, (unsigned long long)cnt[r][c]);
+            atomicAdd(args.count + x * args.ldc + y, (unsigned long long)(cnt[r][c] - past_end));
         }
     }
 }
@@ -435,6 +487,10 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
         if (splits > chunks) splits = chunks;
         if (splits < 1) splits = 1;
         args.split_w = (chunks + splits - 1) / splits * KKW;
+        // a workgroup's counts are 32-bit: fewer than 2^32 sample pairs
+        // (32 per word) per word range
+        const int64_t max_split_w = ((int64_t(1) << 27) - KKW) / KKW * KKW;
+        if (args.split_w > max_split_w) args.split_w = max_split_w;
         splits = (wpad + args.split_w - 1) / args.split_w;
         const int64_t ids = args.ntiles * splits;
         const int64_t max_ids = int64_t(1) << 23;

@@ -103,3 +103,23 @@ def test_device_kendall_strided():
     torch.cuda.synchronize()
     tau, _ = orc.kendall(a.cpu().numpy(), b.cpu().numpy(), nthreads=8)
     assert_same(out.cpu().numpy(), tau)
+
+
+def test_kendall_nan_and_clean_variables_mixed():
+    """Tiles whose variables have no tie and no NaN count with XOR on the P
+    planes; a NaN (here also in the last sample, which is no row's x_i)
+    sends its tiles to the P/N planes.  Both forms meet in one result."""
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((300, 200)).astype(np.float32)   # 3 tiles of variables
+    b = rng.standard_normal((140, 200)).astype(np.float32)
+    a[5, 17] = np.nan
+    b[139, 199] = np.nan
+    print("case kendall mixed clean / nan tiles", flush=True)
+    tau = gt.kendall(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    torch.cuda.synchronize()
+    want, _ = orc.kendall(a, b, nthreads=8)
+    assert_same(tau.cpu().numpy(), want)
+    self_tau = gt.kendall(torch.from_numpy(a).cuda())
+    torch.cuda.synchronize()
+    want_self, _ = orc.kendall(a, a, nthreads=8)
+    assert_same(self_tau.cpu().numpy(), want_self)
