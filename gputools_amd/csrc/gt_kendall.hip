@@ -296,7 +296,7 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
     // staging unit u = tid + 256 q: half u & 1 (words 4 half .. +3 of the
     // chunk), variable (u >> 1) & 127, side/plane u >> 8 (side = sp / 3)
     const uint32_t *src[KUNITS];
-    bool ok[KUNITS];
+    bool ok[KUNITS], inr[KUNITS];
 #pragma unroll
     for (int q = 0; q < KUNITS; ++q) {
         const int u = tid + KTHREADS * q;
@@ -305,9 +305,14 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const int64_t nside = side ? args.nb : args.na;
         const int64_t gv = (side ? y0 : x0) + v;
         // E words are loaded for a side with a tied variable: E & E needs
-        // both sides, the XNOR form's check needs either
-        ok[q] = gv < nside && (plane < 2 || (side ? tie_y : tie_x));
-        src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (ok[q] ? gv : 0)) * args.wpad +
+        // both sides, the XNOR form's check needs either.  N words are loaded
+        // ahead only in a tile with a NaN, and for the first variable (its
+        // P | N | E marks the sample pairs); other chunks that need them
+        // load them when their E bits show up.
+        inr[q] = gv < nside;
+        ok[q] = inr[q] && (plane == 0 || (plane == 1 && (nan_tile || (side == 0 && v == 0))) ||
+                           (plane == 2 && (side ? tie_y : tie_x)));
+        src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (inr[q] ? gv : 0)) * args.wpad +
                  half * 4;
     }
     uint4 stage[KUNITS];
@@ -342,6 +347,26 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const uint4 e0 = stage[2], e1 = stage[5];
         const bool e_here = (e0.x | e0.y | e0.z | e0.w | e1.x | e1.y | e1.z | e1.w) != 0u;
         const bool any_e = __syncthreads_or(e_here) != 0;
+        if (any_e && !nan_tile) {
+            // this chunk counts P and N: load its N words (units 1 and 4)
+            uint4 sn[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = 1 + 3 * h;
+                sn[h] = inr[q] ? *reinterpret_cast<const uint4 *>(src[q] + w) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int u = tid + KTHREADS * (1 + 3 * h);
+                const int sp = u >> 8, v = (u >> 1) & (KBM - 1), half = u & 1;
+                uint32_t *d = &lds[sp / 3][sp % 3][half * 4][v];
+                d[0 * KBM] = sn[h].x;
+                d[1 * KBM] = sn[h].y;
+                d[2 * KBM] = sn[h].z;
+                d[3 * KBM] = sn[h].w;
+            }
+            __syncthreads();
+        }
         if (w + KKW < w_end) gload(w + KKW);
         if (!any_e && !nan_tile) {
             // the chunk's bits past row ends, from the first variable (with
