@@ -3,8 +3,10 @@
 // gputools' binary_kernel_same (inst/cuda/distance.cu:366-415) compares the
 // FP32 values element by element and counts in FP32.  Here each row is first
 // packed into 32-bit words (bit c of word w = x[32w + c] != 0, one wave ballot
-// per 64 features), then a tile kernel counts, per pair and word,
-// popcount(a ^ b) (on in exactly one row) and popcount(a | b) (on in either).
+// per 64 features), with the row's number of set bits in one more word, and a
+// tile kernel counts popcount(a & b) per pair and word (one AND and one
+// v_bcnt_u32_b32, which adds to an accumulator).  From the row counts pa, pb:
+//   on in exactly one row = pa + pb - 2 (a & b),  on in either = pa + pb - (a & b).
 // The counts are exact integers; the ratio uses the correctly rounded FP32
 // divide, which equals the FP32 rounding of the FP64 ratio R computes
 // (double rounding of a quotient from binary64 to binary32 is innocuous), so
@@ -31,14 +33,17 @@ __global__ __launch_bounds__(256) void binary_pack_kernel(const float *__restric
     for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < n; row += (int64_t)gridDim.x * 4) {
         const float *xr = x + row * ldx;
         uint32_t *br = bits + row * ldw;
+        uint32_t total = 0;
         for (int64_t f0 = 0; f0 < k; f0 += 64) {
             const int64_t f = f0 + lane;
             const bool on = (f < k) && (xr[f] != 0.0f);
             const unsigned long long m = __ballot(on);
+            total += (uint32_t)__popcll(m);
             const int64_t w = f0 >> 5;
             if (lane == 0) br[w] = (uint32_t)(m & 0xffffffffull);
             if (lane == 1 && w + 1 < words) br[w + 1] = (uint32_t)(m >> 32);
         }
+        if (lane == 0) br[words] = total;   // set bits of the row
     }
 }
 
@@ -47,7 +52,7 @@ struct BinaryArgs {
     int64_t na;
     const uint32_t *b;  // row side (j)
     int64_t nb;
-    int64_t words, ldw;
+    int64_t words, ldw; // data words per row; row stride (words + 1: the row's bit count)
     float *out;
     int64_t ldd;
     int64_t tiles_a, tile_offset, tiles_this;
@@ -69,11 +74,11 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
     }
     const int64_t i0 = (int64_t)I * BBM, j0 = (int64_t)J * BBM;
 
-    int nx[8][8], no[8][8];
+    uint32_t nand[8][8];
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) nx[r][c] = no[r][c] = 0;
+        for (int c = 0; c < 8; ++c) nand[r][c] = 0;
 
     for (int64_t wb = 0; wb < args.words; wb += BBK) {
         __syncthreads();
@@ -87,8 +92,9 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
             lds[1][wc][row] = (gj < args.nb && w < args.words) ? args.b[gj * args.ldw + w] : 0u;
         }
         __syncthreads();
+        const int nw = (args.words - wb) < BBK ? (int)(args.words - wb) : BBK;
 #pragma unroll 2
-        for (int wc = 0; wc < BBK; ++wc) {
+        for (int wc = 0; wc < nw; ++wc) {
             const uint4 c0 = *reinterpret_cast<const uint4 *>(&lds[0][wc][tx * 4]);
             const uint4 c1 = *reinterpret_cast<const uint4 *>(&lds[0][wc][64 + tx * 4]);
             const uint4 r0 = *reinterpret_cast<const uint4 *>(&lds[1][wc][ty * 4]);
@@ -99,10 +105,20 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
             for (int r = 0; r < 8; ++r)
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    nx[r][c] += __popc(cv[c] ^ rv[r]);
-                    no[r][c] += __popc(cv[c] | rv[r]);
+                    uint32_t s = nand[r][c];
+                    s += (uint32_t)__popc(cv[c] & rv[r]);
+                    asm("" : "+v"(s));   // keeps v_bcnt_u32_b32 accumulating
+                    nand[r][c] = s;
                 }
         }
+    }
+    uint32_t pc[8], pr[8];   // set bits of the lane's rows on each side
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int64_t i = i0 + (u < 4 ? tx * 4 + u : 64 + tx * 4 + (u - 4));
+        const int64_t j = j0 + (u < 4 ? ty * 4 + u : 64 + ty * 4 + (u - 4));
+        pc[u] = i < args.na ? args.a[i * args.ldw + args.words] : 0u;
+        pr[u] = j < args.nb ? args.b[j * args.ldw + args.words] : 0u;
     }
 
 #pragma unroll
@@ -114,7 +130,9 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
             const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
             if (i >= args.na) continue;
             // R: no bit on in either row -> 0; else (#on in one)/(#on in either)
-            const float v = (no[r][c] == 0) ? 0.0f : (float)nx[r][c] / (float)no[r][c];
+            const uint32_t both = pc[c] + pr[r];
+            const uint32_t nx = both - 2u * nand[r][c], no = both - nand[r][c];
+            const float v = (no == 0) ? 0.0f : (float)nx / (float)no;
             if (LAYOUT == OUT_PACKED) {
                 if (i > j) args.out[packed_base(args.na, j) + i] = v;
             } else if (LAYOUT == OUT_FULL) {
@@ -134,6 +152,8 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
 }  // namespace
 
 int64_t binary_words(int64_t k) { return (k + 31) / 32; }
+
+int64_t binary_ldw(int64_t k) { return binary_words(k) + 1; }
 
 hipError_t launch_binary_pack(const float *x, int64_t n, int64_t k, int64_t ldx, uint32_t *bits,
                               int64_t ldw, hipStream_t stream)

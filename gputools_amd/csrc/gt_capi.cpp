@@ -96,7 +96,7 @@ int workspace_bytes(int64_t n, int64_t k, int method, int precision, size_t *byt
     size_t b = 0;
     if (k > 0 && n > 1) {
         if (method == GT_BINARY)
-            b = round_up((size_t)n * (size_t)gt::binary_words(k) * sizeof(uint32_t), 256);
+            b = round_up((size_t)n * (size_t)gt::binary_ldw(k) * sizeof(uint32_t), 256);
         else if (precision != GT_PREC_FP32)
             b = gt::gram_workspace_bytes(n, k, precision);
     }
@@ -154,9 +154,9 @@ int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, floa
             return GT_OK;
         }
         uint32_t *bits = static_cast<uint32_t *>(ws);
-        const int64_t words = gt::binary_words(k);
-        GT_HIP(gt::launch_binary_pack(x, n, k, ldx, bits, words, s));
-        GT_HIP(gt::launch_binary(glayout, bits, n, bits, n, words, words, out, ldd, s));
+        const int64_t words = gt::binary_words(k), ldw = gt::binary_ldw(k);
+        GT_HIP(gt::launch_binary_pack(x, n, k, ldx, bits, ldw, s));
+        GT_HIP(gt::launch_binary(glayout, bits, n, bits, n, words, ldw, out, ldd, s));
         return GT_OK;
     }
     if (precision != GT_PREC_FP32) {
@@ -589,17 +589,17 @@ int gt_dist_cross(const float *a, int64_t na, int64_t lda, const float *b, int64
         return GT_OK;
     }
     if (method == GT_BINARY) {
-        const int64_t words = gt::binary_words(k);
-        const size_t abytes = round_up((size_t)na * words * sizeof(uint32_t), 256);
-        const size_t bbytes = round_up((size_t)nb * words * sizeof(uint32_t), 256);
+        const int64_t words = gt::binary_words(k), ldw = gt::binary_ldw(k);
+        const size_t abytes = round_up((size_t)na * ldw * sizeof(uint32_t), 256);
+        const size_t bbytes = round_up((size_t)nb * ldw * sizeof(uint32_t), 256);
         Scratch ws;
         if ((s = ws.acquire(abytes + bbytes, hs)) != GT_OK) return s;
         uint32_t *abits = static_cast<uint32_t *>(ws.ptr);
         uint32_t *bbits = reinterpret_cast<uint32_t *>(static_cast<char *>(ws.ptr) + abytes);
-        hipError_t e = gt::launch_binary_pack(a, na, k, lda, abits, words, hs);
-        if (e == hipSuccess) e = gt::launch_binary_pack(b, nb, k, ldb, bbits, words, hs);
+        hipError_t e = gt::launch_binary_pack(a, na, k, lda, abits, ldw, hs);
+        if (e == hipSuccess) e = gt::launch_binary_pack(b, nb, k, ldb, bbits, ldw, hs);
         if (e == hipSuccess)
-            e = gt::launch_binary(gt::OUT_CROSS, abits, na, bbits, nb, words, words, out, ldd, hs);
+            e = gt::launch_binary(gt::OUT_CROSS, abits, na, bbits, nb, words, ldw, out, ldd, hs);
         const int r = ws.release();
         if (e != hipSuccess) return hip_fail(e, "binary cross launch");
         return r;

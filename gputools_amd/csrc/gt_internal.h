@@ -33,6 +33,7 @@ hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t 
 
 // Binary method: rows -> bit rows (x != 0), then popcount tiles.
 int64_t binary_words(int64_t k);
+int64_t binary_ldw(int64_t k);   // packed row stride: the words and the row's bit count
 hipError_t launch_binary_pack(const float *x, int64_t n, int64_t k, int64_t ldx,
                               uint32_t *bits, int64_t ldw, hipStream_t stream);
 hipError_t launch_binary(int layout, const uint32_t *a, int64_t na, const uint32_t *b,
