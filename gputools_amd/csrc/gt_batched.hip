@@ -9,8 +9,9 @@
 // workgroups strides through it (workgroup b takes entries b, b + G, ...),
 // so the launch needs no counters or resets and can be captured in a HIP graph
 // and replayed.  Each tile is a BM x BM block of pairs computed with exact FP32
-// differences through LDS (the plain-path arithmetic, two-level summation),
-// written in R 'dist' order into the set's own packed output.
+// differences through LDS (the plain-path arithmetic, two-level summation over
+// 16-feature chunks, the next chunk's loads in registers while one is
+// computed), written in R 'dist' order into the set's own packed output.
 //
 // Tile shapes (gt_batched_tile_count): 1 = 32 x 32, 2 = 64 x 64, 3 = 128 x 128
 // pairs per tile, all with 256 threads; the autotuner picks one per workload.
@@ -21,7 +22,7 @@ namespace gt {
 namespace {
 
 constexpr int BTHREADS = 256;
-constexpr int BKC = 32;         // features per LDS chunk
+constexpr int BKC = 16;         // features per LDS chunk
 
 struct BatchedArgs {
     const float *x;
@@ -42,6 +43,17 @@ __device__ __forceinline__ void bterm(float &part, float a, float b)
     } else {
         part = fmaxf(part, fabsf(a - b));
     }
+}
+
+// Index within a BM-point tile side of element m of thread t's TM points:
+// for TM = 8 two runs of four (t * 4 and BM / 2 + t * 4), so that the 16
+// threads of a side read consecutive 16 B pieces of an LDS row; for smaller
+// TM one run of TM.
+template <int BM, int TM>
+__device__ __forceinline__ int side_index(int t, int m)
+{
+    if (TM == 8) return (m >> 2) * (BM / 2) + t * 4 + (m & 3);
+    return t * TM + m;
 }
 
 template <int OP, int BM>
@@ -68,31 +80,47 @@ __global__ __launch_bounds__(BTHREADS) void batched_kernel(BatchedArgs args)
         float *out = args.out + args.out_off[s];
         const int64_t i0 = (int64_t)I * BM, j0 = (int64_t)J * BM;
 
-        float acc[TM][TM], part[TM][TM];
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-            for (int b = 0; b < TM; ++b) acc[a][b] = part[a][b] = 0.0f;
-
-        for (int64_t kb = 0; kb < k; kb += BKC) {
-            __syncthreads();
+        // global -> register staging, one chunk ahead of the LDS chunk
+        float sa[LOADS], sb[LOADS];
+        auto gload = [&](int64_t kb) {
 #pragma unroll
             for (int q = 0; q < LOADS; ++q) {
                 const int u = tid + BTHREADS * q;
                 const int row = u / BKC, kc = u % BKC;
                 const int64_t kk = kb + kc;
                 const int64_t ri = i0 + row, rj = j0 + row;
-                as_[kc][row] = (rj < n && kk < k) ? x[rj * k + kk] : 0.0f;   // A side: j
-                bs_[kc][row] = (ri < n && kk < k) ? x[ri * k + kk] : 0.0f;   // B side: i
+                sa[q] = (rj < n && kk < k) ? x[rj * k + kk] : 0.0f;   // A side: j
+                sb[q] = (ri < n && kk < k) ? x[ri * k + kk] : 0.0f;   // B side: i
+            }
+        };
+
+        float acc[TM][TM], part[TM][TM];
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b) acc[a][b] = part[a][b] = 0.0f;
+
+        gload(0);
+        for (int64_t kb = 0; kb < k; kb += BKC) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < LOADS; ++q) {
+                const int u = tid + BTHREADS * q;
+                const int row = u / BKC, kc = u % BKC;
+                as_[kc][row] = sa[q];
+                bs_[kc][row] = sb[q];
             }
             __syncthreads();
+            if (kb + BKC < k) gload(kb + BKC);
+            // the chunk's real features only (a set with d = 8 fills half)
+            const int nk = (k - kb) < BKC ? (int)(k - kb) : BKC;
 #pragma unroll 4
-            for (int kc = 0; kc < BKC; ++kc) {
+            for (int kc = 0; kc < nk; ++kc) {
                 float av[TM], bv[TM];
 #pragma unroll
                 for (int m = 0; m < TM; ++m) {
-                    av[m] = as_[kc][ty * TM + m];
-                    bv[m] = bs_[kc][tx * TM + m];
+                    av[m] = as_[kc][side_index<BM, TM>(ty, m)];
+                    bv[m] = bs_[kc][side_index<BM, TM>(tx, m)];
                 }
 #pragma unroll
                 for (int a = 0; a < TM; ++a)
@@ -111,12 +139,12 @@ __global__ __launch_bounds__(BTHREADS) void batched_kernel(BatchedArgs args)
         }
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
-            const int64_t j = j0 + ty * TM + a;
+            const int64_t j = j0 + side_index<BM, TM>(ty, a);
             if (j >= n) continue;
             const int64_t base = packed_base(n, j);
 #pragma unroll
             for (int b = 0; b < TM; ++b) {
-                const int64_t i = i0 + tx * TM + b;
+                const int64_t i = i0 + side_index<BM, TM>(tx, b);
                 if (i < n && i > j) {
                     const float v = (OP == OP_MAXIMUM) ? part[a][b]
                                     : (OP == OP_EUCLID) ? sqrtf(acc[a][b]) : acc[a][b];
