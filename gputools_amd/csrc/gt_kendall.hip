@@ -14,9 +14,10 @@
 // Here the comparisons are made once per variable.  A planes pass encodes
 // each sample pair (i, j), i < j, of a variable as three bits: P (x_j > x_i),
 // N (x_j < x_i) and E (x_j == x_i); a NaN on either side clears all three.
-// A wave compares x_i with 64 consecutive x_j at once (four such loads in
-// flight) and __ballot turns the comparisons into bit words, which
-// v_writelane collects across lanes for coalesced 256-byte stores.  The sample pairs of row i take
+// A wave compares x_i with 64 consecutive x_j at once (the loads of up to 32
+// such steps issued together) and __ballot turns the comparisons into bit
+// words, which v_writelane collects across lanes for coalesced 256-byte
+// stores.  The sample pairs of row i take
 // ceil((n - 1 - i) / 32) words (the tail bits are zero), so
 //   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
 //                      + popc(Ex & Ey)
