@@ -123,6 +123,16 @@ def _valu_roof(flops, ms):
             "frac": round(tf / 157.3, 4)}
 
 
+def _roof(flops, hbm_bytes, ms):
+    # the binding resource: FP32 VALU at 157.3 TFLOP/s or HBM at 8 TB/s,
+    # whichever takes longer for this work
+    if flops / 157.3e12 >= hbm_bytes / 8e12:
+        return _valu_roof(flops, ms)
+    gbs = hbm_bytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(gbs / 8000.0, 4)}
+
+
 def path_methods(torch, gt, wl, steps, warmup, out):
     # all six methods, 8192 x 256 (FP32 VALU kernels; binary on bit rows)
     n, k = 8192, 256
@@ -190,8 +200,9 @@ def path_backward(torch, gt, wl, steps, warmup, out):
             yy = y.detach().requires_grad_(True)
             gt.mds_stress(yy, delta).backward()
         ms = _timed(step, steps, warmup, torch)
+        # bytes: delta read by both owners of every pair (4 B each time)
         mds["q=%d" % q] = {"ms_forward_backward": round(ms, 4),
-                           "roofline": _valu_roof(2.0 * pairs * 5 * q, ms)}
+                           "roofline": _roof(2.0 * pairs * 5 * q, 2.0 * pairs * 4, ms)}
     out["mds_stress_16384"] = mds
 
 

@@ -108,6 +108,25 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
     // the 64 x 64 block of G is element e = tid + 256 q; below the diagonal
     // (i > j) the packed index runs along i, above it along j
     const int js = blockIdx.z, JS = gridDim.z;
+    // G of tile j0 into registers: loaded one tile ahead, so its latency
+    // overlaps the previous tile's work
+    float gq[(BI * BJ) / BTH];
+    auto gfetch = [&](int64_t jt) {
+        const bool up = jt > i0;
+#pragma unroll
+        for (int q = 0; q < (BI * BJ) / BTH; ++q) {
+            const int e = tid + BTH * q;
+            const int li = up ? (e >> 6) : (e & 63), lj = up ? (e & 63) : (e >> 6);
+            const int64_t i = i0 + li, j = jt + lj;
+            float g = 0.0f;
+            if (i < n && j < n && i != j) {
+                const int64_t hi = i > j ? i : j, lo = i > j ? j : i;
+                g = args.g[packed_base(n, lo) + hi];
+            }
+            gq[q] = g;
+        }
+    };
+    if ((int64_t)js * BJ < n) gfetch((int64_t)js * BJ);
     for (int64_t j0 = (int64_t)js * BJ; j0 < n; j0 += (int64_t)JS * BJ) {
         const bool upper = j0 > i0;   // every i < j in this tile
         // ---- d_ij for the 64 x 64 tile, exact FP32 differences ------------
@@ -124,7 +143,10 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
                 const int row = e / DK, kc = e % DK;
                 const int64_t kk = kb + kc;
                 const int64_t ri = i0 + row, rj = j0 + row;
-                xi_s[kc][row] = (ri < n && kk < k) ? args.x[ri * ldx + kk] : 0.0f;
+                // with one chunk (k <= DK) the owned rows' chunk is the same
+                // for every tile: staged on the first only
+                if (k > DK || j0 == (int64_t)js * BJ)
+                    xi_s[kc][row] = (ri < n && kk < k) ? args.x[ri * ldx + kk] : 0.0f;
                 xj_s[kc][row] = (rj < n && kk < k) ? args.x[rj * ldx + kk] : 0.0f;
             }
             __syncthreads();
@@ -161,14 +183,9 @@ __global__ __launch_bounds__(BTH) void backward_kernel(BwArgs args)
         for (int q = 0; q < (BI * BJ) / BTH; ++q) {
             const int e = tid + BTH * q;
             const int li = upper ? (e >> 6) : (e & 63), lj = upper ? (e & 63) : (e >> 6);
-            const int64_t i = i0 + li, j = j0 + lj;
-            float g = 0.0f;
-            if (i < n && j < n && i != j) {
-                const int64_t hi = i > j ? i : j, lo = i > j ? j : i;
-                g = args.g[packed_base(n, lo) + hi];
-            }
-            w_s[li][lj] = g;
+            w_s[li][lj] = gq[q];
         }
+        if (j0 + (int64_t)JS * BJ < n) gfetch(j0 + (int64_t)JS * BJ);
         __syncthreads();
         float w_r[4][4];
 #pragma unroll
