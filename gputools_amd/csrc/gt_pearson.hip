@@ -163,8 +163,13 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
     typedef typename Split<SPLIT>::vec vec;
     constexpr int NT = Split<SPLIT>::NT;
     constexpr int STAGE = 2 * NT * MTERM;
-    __shared__ __attribute__((aligned(1024))) unsigned char smem[MNSTAGE * STAGE + 2 * MBM * 4];
-    float *iA = reinterpret_cast<float *>(smem + MNSTAGE * STAGE);
+    // after the main loop the stages hold each wave's transpose region for
+    // the mirrored stores: 64 rows of MTRL floats
+    constexpr int MTRL = 36;
+    constexpr int TRBYTES = (MTHREADS / 64) * 64 * MTRL * 4;
+    constexpr int SBYTES = MNSTAGE * STAGE > TRBYTES ? MNSTAGE * STAGE : TRBYTES;
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[SBYTES + 2 * MBM * 4];
+    float *iA = reinterpret_cast<float *>(smem + SBYTES);
     float *iB = iA + MBM;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -306,13 +311,36 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
     // SYM: off-diagonal tiles store (x, y) and (y, x); diagonal tiles store
     // x >= y and mirror x > y, so the two halves are the same values
     const bool diag = SYM && (I == J);
+    // whole off-diagonal tiles mirror through LDS: each wave writes its 64 x
+    // 32 block (one output, one a) transposed into its own region, then
+    // stores it back as rows of 32 consecutive x (128 B per row and store),
+    // instead of 16-byte pieces of 32 rows
+    const bool fastm = SYM && VEC && !diag && x0 + MBM <= na && y0 + MBM <= nb;
+    float *const tr = reinterpret_cast<float *>(smem) + wave * 64 * MTRL;
+    // lane's four x (8 rq + 4 (lane >> 5) ..) of row (lane & 31) + 32 b
+    auto tr_put = [&](int rq, int b, const float (&v)[4]) {
+        *reinterpret_cast<float4 *>(tr + ((lane & 31) + 32 * b) * MTRL + 8 * rq + 4 * (lane >> 5)) =
+            make_float4(v[0], v[1], v[2], v[3]);
+    };
+    // the wave's transposed 64 x 32 block out as rows of 32 x
+    auto tr_store = [&](float *out, int a) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int it = 0; it < 8; ++it) {
+            const int ly = 8 * it + (lane >> 3), lx = (lane & 7) * 4;
+            const float4 v = *reinterpret_cast<const float4 *>(tr + ly * MTRL + lx);
+            *reinterpret_cast<float4 *>(out + (y0 + wc * 64 + ly) * ldo + x0 + wr * 64 + a * 32 + lx) = v;
+        }
+    };
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (fastm) __syncthreads();   // the region's earlier reads (or the main loop's) are done
+        float tva[4][2][4];
 #pragma unroll
         for (int rq = 0; rq < 4; ++rq) {
             // the four x rows of this lane: lx0 .. lx0 + 3
             const int lx0 = wr * 64 + a * 32 + 8 * rq + 4 * (lane >> 5);
-            float cv[2][4], tv[2][4];
+            float cv[2][4];
+            float (&tv)[2][4] = tva[rq];
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
                 const int64_t x = x0 + lx0 + r4;
@@ -333,7 +361,10 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
                     }
                 }
             }
-            if (SYM) {
+            if (fastm) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b) tr_put(rq, b, cv[b]);
+            } else if (SYM) {
                 // (y, x) for the four x of this lane: one 16-byte store per
                 // output where the rows allow it
 #pragma unroll
@@ -360,6 +391,27 @@ __global__ __launch_bounds__(MTHREADS) void pmcc_tile_kernel(PmccTileArgs args)
                 }
             }
         }
+        if (fastm) {
+            __syncthreads();
+            tr_store(args.corr, a);
+            __syncthreads();
+#pragma unroll
+            for (int rq = 0; rq < 4; ++rq)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) tr_put(rq, b, tva[rq][b]);
+            __syncthreads();
+            tr_store(args.ts, a);
+        }
+    }
+    if (fastm) {
+        // pairs is one value: the wave's mirrored 64 x 64 block as rows
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int ly = 4 * it + (lane >> 4), lx = (lane & 15) * 4;
+            *reinterpret_cast<float4 *>(args.pairs + (y0 + wc * 64 + ly) * ldo + x0 + wr * 64 + lx) =
+                make_float4(fp, fp, fp, fp);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

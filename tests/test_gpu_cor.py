@@ -99,6 +99,26 @@ def test_device_pmcc_all_precisions(precision, self_case):
         np.testing.assert_array_equal(got, got.T)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_device_pmcc_self_whole_offdiagonal_tiles(precision):
+    """600 variables: tile (1, 0) of the symmetric schedule is a whole
+    256 x 256 tile, whose mirrored half goes out through LDS as rows."""
+    rng = np.random.default_rng(23)
+    a = _data(rng, 300, 600, shift=0.5).T.copy()
+    print("case pmcc self 600 vars %s" % precision, flush=True)
+    pairs, corr, ts = gt.pmcc(torch.from_numpy(a).cuda(), precision=precision)
+    torch.cuda.synchronize()
+    ref_p, ref_c = orc.pmcc(a, a, nthreads=8)
+    got = corr.cpu().numpy()
+    print("max abs error %.3g" % np.abs(got - ref_c).max(), flush=True)
+    assert np.abs(got - ref_c).max() <= TOL
+    np.testing.assert_array_equal(got, got.T)
+    t = ts.cpu().numpy()
+    np.testing.assert_array_equal(t, t.T)
+    assert_bits_equal(t, orc.signif(pairs.cpu().numpy(), got))
+    assert (pairs.cpu().numpy() == 300).all()
+
+
 def test_device_pmcc_strided_output_and_input():
     rng = np.random.default_rng(22)
     base = torch.from_numpy(_data(rng, 300, 150).T.copy()).cuda()
