@@ -432,6 +432,7 @@ struct PairwiseArgs {
     int64_t ldo;
     int64_t tiles_b;
     int64_t tile_offset, tiles_this;
+    bool vec;   // ldo and the output pointers allow 16-byte stores
 };
 
 // A sample of a pair counts when both values are present (not NaN): counts
@@ -615,6 +616,39 @@ __global__ __launch_bounds__(WTHREADS) void pmcc_pairwise_kernel(PairwiseArgs ar
 
     // epilogue: no samples gives the reference's 0 / (0 - 1) = -0
     const bool mirror = SYM && (I != J);
+    float rv[4][4], tv[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float c = cnt[i][j];
+            rv[i][j] = (c == 0.0f) ? -0.0f : qab[i][j] / (sqrtf(qa[i][j]) * sqrtf(qb[i][j]));
+            tv[i][j] = t_score<true>(c, rv[i][j]);
+        }
+    // a lane's 4 x 4 pairs are four consecutive y per x (and four
+    // consecutive x per y): 16-byte stores where the rows allow them
+    const bool vec = args.vec && x0 + WBM <= args.na && y0 + WBM <= args.nb;
+    if (vec) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t o = (x0 + ty * 4 + i) * args.ldo + y0 + tx * 4;
+            *reinterpret_cast<float4 *>(args.corr + o) = make_float4(rv[i][0], rv[i][1], rv[i][2], rv[i][3]);
+            *reinterpret_cast<float4 *>(args.ts + o) = make_float4(tv[i][0], tv[i][1], tv[i][2], tv[i][3]);
+            *reinterpret_cast<float4 *>(args.pairs + o) =
+                make_float4(cnt[i][0], cnt[i][1], cnt[i][2], cnt[i][3]);
+        }
+        if (mirror) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t o = (y0 + tx * 4 + j) * args.ldo + x0 + ty * 4;
+                *reinterpret_cast<float4 *>(args.corr + o) = make_float4(rv[0][j], rv[1][j], rv[2][j], rv[3][j]);
+                *reinterpret_cast<float4 *>(args.ts + o) = make_float4(tv[0][j], tv[1][j], tv[2][j], tv[3][j]);
+                *reinterpret_cast<float4 *>(args.pairs + o) =
+                    make_float4(cnt[0][j], cnt[1][j], cnt[2][j], cnt[3][j]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int64_t x = x0 + ty * 4 + i;
@@ -623,18 +657,15 @@ __global__ __launch_bounds__(WTHREADS) void pmcc_pairwise_kernel(PairwiseArgs ar
         for (int j = 0; j < 4; ++j) {
             const int64_t y = y0 + tx * 4 + j;
             if (y >= args.nb) continue;
-            const float c = cnt[i][j];
-            const float r = (c == 0.0f) ? -0.0f : qab[i][j] / (sqrtf(qa[i][j]) * sqrtf(qb[i][j]));
-            const float tsv = t_score<true>(c, r);
             const int64_t o = x * args.ldo + y;
-            args.corr[o] = r;
-            args.ts[o] = tsv;
-            args.pairs[o] = c;
+            args.corr[o] = rv[i][j];
+            args.ts[o] = tv[i][j];
+            args.pairs[o] = cnt[i][j];
             if (mirror) {
                 const int64_t om = y * args.ldo + x;
-                args.corr[om] = r;
-                args.ts[om] = tsv;
-                args.pairs[om] = c;
+                args.corr[om] = rv[i][j];
+                args.ts[om] = tv[i][j];
+                args.pairs[om] = cnt[i][j];
             }
         }
     }
@@ -814,6 +845,8 @@ hipError_t launch_pmcc(int use, int precision, const float *a, int64_t na, int64
         args.ts = ts;
         args.pairs = pairs;
         args.ldo = ldo;
+        args.vec = (ldo % 4 == 0) && ((reinterpret_cast<uintptr_t>(corr) | reinterpret_cast<uintptr_t>(ts) |
+                                       reinterpret_cast<uintptr_t>(pairs)) & 15u) == 0;
         const int64_t ta = (na + WBM - 1) / WBM, tb = (nb + WBM - 1) / WBM;
         args.tiles_b = tb;
         const int64_t total = same ? ta * (ta + 1) / 2 : ta * tb;
