@@ -876,9 +876,14 @@ hipError_t launch_pmcc(int use, int precision, const float *a, int64_t na, int64
     const float fpairs = (float)dim;
     if (precision == 0) {
         // corr[x * ldo + y] = z_x . z_y: the plain cross tile with x on its
-        // row side and y on its column side
-        e = launch_plain(OP_DOT, OUT_CROSS, w.zb, nb, dim, w.za, na, dim, dim, 2.0f, corr, ldo,
-                         stream);
+        // row side and y on its column side; with a == b the symmetric
+        // layout computes the lower tiles and mirrors them
+        if (same)
+            e = launch_plain(OP_DOT, OUT_FULL, w.za, na, dim, w.za, na, dim, dim, 2.0f, corr, ldo,
+                             stream);
+        else
+            e = launch_plain(OP_DOT, OUT_CROSS, w.zb, nb, dim, w.za, na, dim, dim, 2.0f, corr, ldo,
+                             stream);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(pmcc_finish_kernel, dim3(grid_for(na * nb, 256)), dim3(256), 0, stream,
                            corr, ts, pairs, na, nb, ldo, fpairs);

@@ -248,7 +248,10 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
                     args.out[j * args.ldd + i] = v;
                     args.out[i * args.ldd + j] = v;
                 } else if (i == j) {
-                    args.out[i * args.ldd + i] = 0.0f;
+                    // a distance is 0 there; the Pearson dot product is z . z
+                    args.out[i * args.ldd + i] =
+                        (OP == OP_DOT) ? Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk)
+                                       : 0.0f;
                 }
             } else {
                 args.out[j * args.ldd + i] = Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk);
