@@ -43,6 +43,7 @@ struct PlainArgs {
     int64_t tile_offset;  // first logical tile of this launch
     int64_t tiles_this;   // tiles in this launch
     float p, inv_p, fk;
+    bool vec;             // ldd and out allow 16-byte stores (pitched layouts)
 };
 
 // Canberra chunk modes.  The quotient |a-b| / |a+b| costs an IEEE divide
@@ -229,6 +230,40 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
         }
     }
 
+    // Whole tiles of the pitched layouts (cross, and the off-diagonal tiles
+    // of the symmetric one) store 16 bytes at a time: the lane's i runs
+    // tx*4.. and 64 + tx*4.. along a row j, and its j runs ty*4.. and
+    // 64 + ty*4.. along a mirrored row i.
+    if (LAYOUT != OUT_PACKED && args.vec && i0 + PBM <= args.na && j0 + PBM <= args.nb &&
+        (LAYOUT == OUT_CROSS || I > J)) {
+        float v[8][8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                v[r][c] = Term<OP>::finish((OP == OP_MAXIMUM) ? part[r][c] : acc[r][c],
+                                           cnt[r][c] + clean_terms, args.inv_p, args.fk);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<float4 *>(args.out + j * args.ldd + i0 + h * 64 + tx * 4) =
+                    make_float4(v[r][4 * h], v[r][4 * h + 1], v[r][4 * h + 2], v[r][4 * h + 3]);
+        }
+        if (LAYOUT == OUT_FULL) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    *reinterpret_cast<float4 *>(args.out + i * args.ldd + j0 + h * 64 + ty * 4) =
+                        make_float4(v[4 * h][c], v[4 * h + 1][c], v[4 * h + 2][c], v[4 * h + 3][c]);
+            }
+        }
+        return;
+    }
+
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
@@ -297,6 +332,7 @@ hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t 
     args.k = k;
     args.out = out;
     args.ldd = ldd;
+    args.vec = (ldd % 4 == 0) && ((reinterpret_cast<uintptr_t>(out) & 15u) == 0);
     args.p = p;
     args.inv_p = 1.0f / p;
     args.fk = (float)k;
