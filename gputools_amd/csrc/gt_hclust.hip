@@ -390,19 +390,24 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
                 if constexpr (LDSRES) {
                     // row and column sub are left as they are (sub is dead,
                     // and scans skip dead columns); a dead column c holds a
-                    // stale value, so its new entry is +inf
+                    // stale value, so its new entry is +inf.  A dead row or
+                    // column is never read again: only live c are stored.
                     const float nkc = cntp[c];
                     if (c != cw && c != rw && nkc > 0.0f)
                         v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nkc, val, a.lambda, a.beta);
+                    newp[c] = v;
+                    if (nkc > 0.0f) {
+                        drw[c] = v;
+                        a.d[(int64_t)c * ldd + rw] = v;
+                    }
                 } else {
                     if (c != cw && c != rw)
                         v = lw_update<METHOD>(bot[u], top[u], nr, np, nq, nk[u], val, a.lambda, a.beta);
                     dcw[c] = INFINITY;
                     a.d[(int64_t)c * ldd + cw] = INFINITY;
+                    drw[c] = v;
+                    a.d[(int64_t)c * ldd + rw] = v;
                 }
-                drw[c] = v;
-                a.d[(int64_t)c * ldd + rw] = v;
-                if constexpr (LDSRES) newp[c] = v;
             }
         }
         if constexpr (LDSRES)
