@@ -317,14 +317,18 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
             lds_barrier();
         else
             __syncthreads();
+        // the waves' candidates reduced by wave 0 (the order on (value, tie
+        // key) is total, so the lane order of the reduction does not matter)
+        float v = INFINITY;
+        int r = NO_COL;
+        if (wave == 0) {
+            if (lane < HWAVES) {
+                v = red_v[lane];
+                r = red_i[lane];
+            }
+            wave_argmin(v, r);
+        }
         if (tid == 0) {
-            float v = INFINITY;
-            int r = NO_COL;
-            for (int w = 0; w < HWAVES; ++w)
-                if (red_i[w] != NO_COL && (r == NO_COL || beats(red_v[w], red_i[w], v, r))) {
-                    v = red_v[w];
-                    r = red_i[w];
-                }
             int cw, rw;
             if (r != NO_COL) {
                 rw = r;
