@@ -55,9 +55,13 @@ struct PlainArgs {
 // quotient with the same Inf (x/0) and NaN (0/0, Inf/Inf) results.
 //   CB_CLEAN:  no 0 and no non-finite value -- no term can be NaN, so the
 //              per-pair count is the chunk length, added once per chunk;
-//   CB_RANGED: zeros or non-finite values present -- reciprocal + NaN test;
+//   CB_ZEROS:  zeros but no non-finite value -- the only NaN term is 0/0
+//              (a == b == 0), which max(q, 0) drops (IEEE maxNum returns the
+//              non-NaN operand); the dropped terms are counted per chunk as
+//              popcount(za & zb) of the two rows' 16-bit zero masks;
+//   CB_RANGED: non-finite values present -- reciprocal + NaN test per term;
 //   CB_IEEE:   a tiny or huge finite value present -- IEEE divide.
-enum { CB_CLEAN = 0, CB_RANGED = 1, CB_IEEE = 2 };
+enum { CB_CLEAN = 0, CB_RANGED = 1, CB_IEEE = 2, CB_ZEROS = 3 };
 
 template <int OP, int MODE = CB_IEEE>
 struct Term {
@@ -73,6 +77,8 @@ struct Term {
             part += fabsf(a - b);
         } else if (OP == OP_CANBERRA && MODE == CB_CLEAN) {
             part = fmaf(fabsf(a - b), __builtin_amdgcn_rcpf(fabsf(a + b)), part);
+        } else if (OP == OP_CANBERRA && MODE == CB_ZEROS) {
+            part += fmaxf(fabsf(a - b) * __builtin_amdgcn_rcpf(fabsf(a + b)), 0.0f);
         } else if (OP == OP_CANBERRA && MODE == CB_RANGED) {
             const float q = fabsf(a - b) * __builtin_amdgcn_rcpf(fabsf(a + b));
             const bool ok = (q == q);
@@ -175,12 +181,13 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
     };
 
     __shared__ int cb_flags[PTHREADS / 64];  // canberra: per-wave chunk classification
-    int clean_terms = 0;                     // canberra: features of CB_CLEAN chunks
+    __shared__ uint32_t cb_zero[2][PBM];     // canberra CB_ZEROS: per-row zero masks
+    int clean_terms = 0;                     // canberra: features of CB_CLEAN / CB_ZEROS chunks
 
     gload(0);
     for (int64_t kb = 0; kb < k; kb += PBK) {
         __syncthreads();
-        bool unclean = false, unranged = false;
+        bool unclean = false, unranged = false, nonfin = false;
 #pragma unroll
         for (int q = 0; q < PLOADS; ++q) {
             const int e = tid + PTHREADS * q;
@@ -194,13 +201,15 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
                     const bool mid = (m >= 0x1p-100f) && (m <= 0x1p100f);
                     unclean |= !mid;
                     unranged |= !mid && m != 0.0f && m < __builtin_inff();
+                    nonfin |= !(m < __builtin_inff());
                 }
             }
         }
         if (OP == OP_CANBERRA) {
             const bool wu = __builtin_amdgcn_ballot_w64(unclean) != 0;
             const bool wr = __builtin_amdgcn_ballot_w64(unranged) != 0;
-            if ((tid & 63) == 0) cb_flags[tid >> 6] = (wu ? 1 : 0) | (wr ? 2 : 0);
+            const bool wn = __builtin_amdgcn_ballot_w64(nonfin) != 0;
+            if ((tid & 63) == 0) cb_flags[tid >> 6] = (wu ? 1 : 0) | (wr ? 2 : 0) | (wn ? 4 : 0);
         }
         __syncthreads();
         if (kb + PBK < k) gload(kb + PBK);
@@ -212,6 +221,28 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
                 chunk(std::integral_constant<int, CB_CLEAN>{});
                 clean_terms += PBK;
             } else if (bits == 1) {
+                // zero masks of the 2 x 128 staged rows, one row per thread
+                {
+                    const int side = tid >> 7, row = tid & (PBM - 1);
+                    uint32_t zm = 0;
+#pragma unroll
+                    for (int kc = 0; kc < PBK; ++kc) zm |= (lds[side][kc][row] == 0.0f ? 1u : 0u) << kc;
+                    cb_zero[side][row] = zm;
+                }
+                __syncthreads();
+                chunk(std::integral_constant<int, CB_ZEROS>{});
+                clean_terms += PBK;
+                uint32_t zc[8], zr[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    zc[u] = cb_zero[0][u < 4 ? tx * 4 + u : 64 + tx * 4 + (u - 4)];
+                    zr[u] = cb_zero[1][u < 4 ? ty * 4 + u : 64 + ty * 4 + (u - 4)];
+                }
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) cnt[r][c] -= __builtin_popcount(zc[c] & zr[r]);
+            } else if ((bits & 2) == 0) {
                 chunk(std::integral_constant<int, CB_RANGED>{});
             } else {
                 chunk(std::integral_constant<int, CB_IEEE>{});

@@ -152,12 +152,15 @@ def test_canberra_r_edge_rules_on_gpu():
 @pytest.mark.parametrize("n,k", [(384, 112), (300, 100)])
 def test_canberra_chunk_modes(n, k):
     """Each 16-feature chunk takes the reciprocal path when its values allow it
-    (no 0 / non-finite value: clean; zeros or NaN: reciprocal + NaN test) and
-    the IEEE divide otherwise (a tiny or huge magnitude).  One pair's sum mixes
-    all three; 384 rows fill whole 128-row tiles, 300 leaves a zero-padded one."""
+    (no 0 / non-finite value: clean; zeros: max(q, 0) and zero-mask counts;
+    NaN: reciprocal + NaN test) and the IEEE divide otherwise (a tiny or huge
+    magnitude).  One pair's sum mixes all four; 384 rows fill whole 128-row
+    tiles, 300 leaves a zero-padded one."""
     rng = np.random.default_rng(n + k)
     x = rng.standard_normal((n, k)).astype(np.float32)
     x[:, 0:16] = rng.uniform(0.1, 10.0, (n, 16))                      # clean, positive
+    zb = x[:, 16:32]                                                   # zeros, no NaN
+    zb[rng.random(zb.shape) < 0.3] = 0.0
     blk = x[:, 32:48]                                                  # zeros and NaN
     blk[rng.random(blk.shape) < 0.3] = 0.0
     blk[rng.random(blk.shape) < 0.02] = np.nan
@@ -166,6 +169,8 @@ def test_canberra_chunk_modes(n, k):
     x[:, 64:80] *= np.float32(1e35)                                    # huge
     x[:, 80] = np.float32(2.0 ** -100)                                 # the range bounds
     x[::2, 81] = np.float32(-(2.0 ** 100))
+    zt = x[:, 96:k]                                                    # zeros, and
+    zt[rng.random(zt.shape) < 0.3] = 0.0
     x[3, 96:k] = -x[7, 96:k]                                           # a = -b: Inf term
     print("case canberra chunk modes n=%d k=%d" % (n, k), flush=True)
     got = run(x, "canberra")
