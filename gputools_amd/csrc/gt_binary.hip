@@ -56,6 +56,7 @@ struct BinaryArgs {
     float *out;
     int64_t ldd;
     int64_t tiles_a, tile_offset, tiles_this;
+    bool vec;           // ldd and out allow 16-byte stores (pitched layouts)
 };
 
 template <int LAYOUT>
@@ -121,6 +122,39 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
         pr[u] = j < args.nb ? args.b[j * args.ldw + args.words] : 0u;
     }
 
+    // Whole tiles of the pitched layouts (cross, and the off-diagonal tiles
+    // of the symmetric one) store 16 bytes at a time in both directions.
+    if (LAYOUT != OUT_PACKED && args.vec && i0 + BBM <= args.na && j0 + BBM <= args.nb &&
+        (LAYOUT == OUT_CROSS || I > J)) {
+        float v[8][8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t both = pc[c] + pr[r];
+                const uint32_t nx = both - 2u * nand[r][c], no = both - nand[r][c];
+                v[r][c] = (no == 0) ? 0.0f : (float)nx / (float)no;
+            }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<float4 *>(args.out + j * args.ldd + i0 + h * 64 + tx * 4) =
+                    make_float4(v[r][4 * h], v[r][4 * h + 1], v[r][4 * h + 2], v[r][4 * h + 3]);
+        }
+        if (LAYOUT == OUT_FULL) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    *reinterpret_cast<float4 *>(args.out + i * args.ldd + j0 + h * 64 + ty * 4) =
+                        make_float4(v[4 * h][c], v[4 * h + 1][c], v[4 * h + 2][c], v[4 * h + 3][c]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
@@ -178,6 +212,7 @@ hipError_t launch_binary(int layout, const uint32_t *a, int64_t na, const uint32
     args.ldw = ldw;
     args.out = out;
     args.ldd = ldd;
+    args.vec = (ldd % 4 == 0) && ((reinterpret_cast<uintptr_t>(out) & 15u) == 0);
     const int64_t ta = (na + BBM - 1) / BBM, tb = (nb + BBM - 1) / BBM;
     args.tiles_a = ta;
     const int64_t total = (layout == OUT_CROSS) ? ta * tb : ta * (ta + 1) / 2;

@@ -119,6 +119,26 @@ def test_full_layout_matches_packed_with_pitch(method):
     assert_parity(packed.cpu().numpy(), orc.dist(x, method, p), method)
 
 
+@pytest.mark.parametrize("method", ["euclidean", "maximum", "manhattan", "canberra", "binary",
+                                    "minkowski"])
+def test_full_layout_whole_tiles(method):
+    """384 points with a pitch of 384: the off-diagonal 128 x 128 tiles are
+    whole and store 16 bytes at a time, directly and mirrored."""
+    n, k = 384, 70
+    x = wl.method_input(method, n, k, seed=17)
+    p = 3.0 if method == "minkowski" else 2.0
+    print("case full layout whole tiles", method, flush=True)
+    xt = dev(x)
+    full = gt.dist(xt, method, p, layout="full").cpu().numpy()
+    packed = gt.dist(xt, method, p).cpu().numpy()
+    assert np.array_equal(full, full.T)
+    assert np.all(np.diag(full) == 0.0)
+    ii, jj = np.tril_indices(n, -1)
+    order = np.lexsort((ii, jj))
+    np.testing.assert_array_equal(full[ii[order], jj[order]], packed)
+    assert_parity(packed, orc.dist(x, method, p), method)
+
+
 def test_zero_features_give_nan():
     x = np.zeros((5, 0), np.float32)
     xt = torch.zeros((5, 0), dtype=torch.float32, device="cuda")
