@@ -7,6 +7,7 @@ getDistEnum does, and reject bad arguments before touching the device.
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -20,7 +21,8 @@ HEADER = os.path.join(ROOT, "include", "gputools_amd.h")
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    with open(HEADER) as f:
+        text = f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
     skip = {"defined"}
@@ -37,7 +39,8 @@ def test_library_loads_and_exports_every_declared_symbol():
 
 
 def test_exported_symbols_are_unmangled():
-    out = os.popen("nm -D --defined-only %s" % _lib.LIB_PATH).read()
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
     for name in declared_functions():
         assert re.search(r"\bT %s$" % name, out, flags=re.M), name
 
