@@ -243,8 +243,8 @@ def path_pmcc(torch, gt, wl, steps, warmup, out):
     # pair gets pairs, corr and t-score: 12 B of output, which bounds the
     # "everything" path; its algorithmic work is one 1024-long dot product
     # per unique pair.  pairwise.complete.obs (10% NaN) is per-pair VALU
-    # work: 3 accumulations (count, two sums) and then 2 differences and 3
-    # fused products per sample, 11 flops.
+    # work: one pass of 6 fused products per sample (count, two shifted sums,
+    # three second moments), 12 flops.
     nv, size = 8192, 1024
     a = torch.from_numpy(wl.normal(nv, size, seed=6)).cuda()
     outs = tuple(torch.empty((nv, nv), dtype=torch.float32, device="cuda") for _ in range(3))
@@ -264,7 +264,7 @@ def path_pmcc(torch, gt, wl, steps, warmup, out):
                 steps, warmup, torch)
     cor["pairwise_complete_fp32"] = {
         "ms": round(ms, 4), "pairs_per_s": round(nv * nv / (ms * 1e-3), 1),
-        "roofline": _valu_roof(11.0 * size * nv * (nv + 1) / 2, ms)}
+        "roofline": _valu_roof(12.0 * size * nv * (nv + 1) / 2, ms)}
     out["pmcc_self_8192x1024"] = cor
 
 

@@ -485,135 +485,253 @@ __global__ __launch_bounds__(WTHREADS) void pmcc_pairwise_kernel(PairwiseArgs ar
         }
     };
 
-    float cnt[4][4], s1[4][4], s2[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cnt[i][j] = s1[i][j] = s2[i][j] = 0.0f;
+    float cnt[4][4], qa[4][4], qb[4][4], qab[4][4];
 
-    // pass 1: count, sum x, sum y over the pair's samples.  A chunk without
-    // Inf uses products with 0/1 masks (x * 0 = 0 for finite x): one fma per
-    // sum instead of a select and an add, with the same sums bit for bit; a
-    // chunk with an Inf selects.
-    gload(0);
-    for (int64_t kb = 0; kb < dim; kb += WBK) {
-        __syncthreads();
-        stage();
-        bool inf_here = false;
+    // ---- one pass ---------------------------------------------------------
+    // Each variable is shifted by c, the mean of its present values in the
+    // first chunk, and the pair's count and its five shifted sums come from
+    // one pass: n = sum m_a m_b, Sa = sum a' m_b, Sb = sum m_a b',
+    // Saa = sum a'^2 m_b, Sbb = sum m_a b'^2, Sab = sum a' b' (a' = a - c where
+    // present, else 0).  The centred moments are then Saa - Sa^2 / n, ... .
+    // Where that subtraction cancels (Caa <= Saa / 16: a subset mean far from
+    // c, a constant subset, fewer than two samples) or a sum is not finite
+    // (Inf in the data), the tile runs the two passes below instead.
+    {
+        __shared__ float lds1[3][2][WBK][WLD];   // a', a'^2, presence; [side]
+        gload(0);
+        float csum[2] = {0.0f, 0.0f}, ccnt[2] = {0.0f, 0.0f};
 #pragma unroll
-        for (int q = 0; q < 8; ++q) inf_here = inf_here || __builtin_isinf(ra[q]) || __builtin_isinf(rb[q]);
-        const bool any_inf = __syncthreads_or(inf_here) != 0;
-        if (kb + WBK < dim) gload(kb + WBK);
-        float p1[4][4], p2[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) p1[i][j] = p2[i][j] = 0.0f;
-        if (!any_inf) {
-#pragma unroll 4
-            for (int kc = 0; kc < WBK; ++kc) {
-                const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
-                const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
-                const float av[4] = {a4.x, a4.y, a4.z, a4.w};
-                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
-                float am[4], a0[4], bm[4], b0[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    am[i] = av[i] == av[i] ? 1.0f : 0.0f;
-                    a0[i] = av[i] == av[i] ? av[i] : 0.0f;
-                    bm[i] = bv[i] == bv[i] ? 1.0f : 0.0f;
-                    b0[i] = bv[i] == bv[i] ? bv[i] : 0.0f;
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        cnt[i][j] = fmaf(am[i], bm[j], cnt[i][j]);
-                        p1[i][j] = fmaf(a0[i], bm[j], p1[i][j]);
-                        p2[i][j] = fmaf(b0[j], am[i], p2[i][j]);
-                    }
+        for (int q = 0; q < 8; ++q) {
+            if (fabsf(ra[q]) < INFINITY) {
+                csum[0] += ra[q];
+                ccnt[0] += 1.0f;
             }
-        } else {
-#pragma unroll 4
-            for (int kc = 0; kc < WBK; ++kc) {
-                const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
-                const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
-                const float av[4] = {a4.x, a4.y, a4.z, a4.w};
-                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
-                        cnt[i][j] += m ? 1.0f : 0.0f;
-                        p1[i][j] += m ? av[i] : 0.0f;
-                        p2[i][j] += m ? bv[j] : 0.0f;
-                    }
+            if (fabsf(rb[q]) < INFINITY) {
+                csum[1] += rb[q];
+                ccnt[1] += 1.0f;
             }
         }
+        // the four lanes of a variable are consecutive lanes of one wave
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                s1[i][j] += p1[i][j];
-                s2[i][j] += p2[i][j];
-            }
-    }
-    // the pair's means (0 / 0 = NaN for a pair without samples: its result
-    // is set below)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            s1[i][j] = s1[i][j] / cnt[i][j];
-            s2[i][j] = s2[i][j] / cnt[i][j];
+        for (int h = 0; h < 2; ++h) {
+            csum[h] += __shfl_xor(csum[h], 1, 64);
+            ccnt[h] += __shfl_xor(ccnt[h], 1, 64);
+            csum[h] += __shfl_xor(csum[h], 2, 64);
+            ccnt[h] += __shfl_xor(ccnt[h], 2, 64);
         }
+        const float ca = ccnt[0] > 0.0f ? csum[0] / ccnt[0] : 0.0f;
+        const float cb = ccnt[1] > 0.0f ? csum[1] / ccnt[1] : 0.0f;
 
-    // pass 2: sum dx^2, sum dy^2, sum dx dy
-    float qa[4][4], qb[4][4], qab[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) qa[i][j] = qb[i][j] = qab[i][j] = 0.0f;
-    gload(0);
-    for (int64_t kb = 0; kb < dim; kb += WBK) {
-        __syncthreads();
-        stage();
-        __syncthreads();
-        if (kb + WBK < dim) gload(kb + WBK);
-        float pa2[4][4], pb2[4][4], pab[4][4];
+        float sa[4][4], sb[4][4], saa[4][4], sbb[4][4], sab[4][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) pa2[i][j] = pb2[i][j] = pab[i][j] = 0.0f;
+            for (int j = 0; j < 4; ++j)
+                cnt[i][j] = sa[i][j] = sb[i][j] = saa[i][j] = sbb[i][j] = sab[i][j] = 0.0f;
+        for (int64_t kb = 0; kb < dim; kb += WBK) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const bool pa_ = ra[q] == ra[q], pb_ = rb[q] == rb[q];
+                const float da = pa_ ? ra[q] - ca : 0.0f, db = pb_ ? rb[q] - cb : 0.0f;
+                lds1[0][0][sk + q][sv] = da;
+                lds1[1][0][sk + q][sv] = da * da;
+                lds1[2][0][sk + q][sv] = pa_ ? 1.0f : 0.0f;
+                lds1[0][1][sk + q][sv] = db;
+                lds1[1][1][sk + q][sv] = db * db;
+                lds1[2][1][sk + q][sv] = pb_ ? 1.0f : 0.0f;
+            }
+            __syncthreads();
+            if (kb + WBK < dim) gload(kb + WBK);
+            // the second-moment sums are two-level (chunk partials, then
+            // totals); Sa and Sb, small sums of shifted values that only
+            // enter as Sa^2 / n, accumulate directly
+            float p_aa[4][4], p_bb[4][4], p_ab[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) p_aa[i][j] = p_bb[i][j] = p_ab[i][j] = 0.0f;
 #pragma unroll 2
-        for (int kc = 0; kc < WBK; ++kc) {
-            const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
-            const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
-            const float av[4] = {a4.x, a4.y, a4.z, a4.w};
-            const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+            for (int kc = 0; kc < WBK; ++kc) {
+                const float4 av4 = *reinterpret_cast<const float4 *>(&lds1[0][0][kc][ty * 4]);
+                const float4 aq4 = *reinterpret_cast<const float4 *>(&lds1[1][0][kc][ty * 4]);
+                const float4 am4 = *reinterpret_cast<const float4 *>(&lds1[2][0][kc][ty * 4]);
+                const float4 bv4 = *reinterpret_cast<const float4 *>(&lds1[0][1][kc][tx * 4]);
+                const float4 bq4 = *reinterpret_cast<const float4 *>(&lds1[1][1][kc][tx * 4]);
+                const float4 bm4 = *reinterpret_cast<const float4 *>(&lds1[2][1][kc][tx * 4]);
+                const float av[4] = {av4.x, av4.y, av4.z, av4.w}, aq[4] = {aq4.x, aq4.y, aq4.z, aq4.w};
+                const float am[4] = {am4.x, am4.y, am4.z, am4.w};
+                const float bv[4] = {bv4.x, bv4.y, bv4.z, bv4.w}, bq[4] = {bq4.x, bq4.y, bq4.z, bq4.w};
+                const float bm[4] = {bm4.x, bm4.y, bm4.z, bm4.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        cnt[i][j] = fmaf(am[i], bm[j], cnt[i][j]);   // exact: small integers
+                        sa[i][j] = fmaf(av[i], bm[j], sa[i][j]);
+                        sb[i][j] = fmaf(am[i], bv[j], sb[i][j]);
+                        p_aa[i][j] = fmaf(aq[i], bm[j], p_aa[i][j]);
+                        p_bb[i][j] = fmaf(am[i], bq[j], p_bb[i][j]);
+                        p_ab[i][j] = fmaf(av[i], bv[j], p_ab[i][j]);
+                    }
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
-                    const float d = m ? av[i] - s1[i][j] : 0.0f;
-                    const float e = m ? bv[j] - s2[i][j] : 0.0f;
-                    pa2[i][j] = fmaf(d, d, pa2[i][j]);
-                    pb2[i][j] = fmaf(e, e, pb2[i][j]);
-                    pab[i][j] = fmaf(d, e, pab[i][j]);
+                    saa[i][j] += p_aa[i][j];
+                    sbb[i][j] += p_bb[i][j];
+                    sab[i][j] += p_ab[i][j];
                 }
         }
+        bool ill = false;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                qa[i][j] += pa2[i][j];
-                qb[i][j] += pb2[i][j];
-                qab[i][j] += pab[i][j];
+                const float n = cnt[i][j];
+                // Sa Sb / n with the product first, so (x, y) and (y, x) round alike
+                qa[i][j] = saa[i][j] - sa[i][j] * sa[i][j] / n;
+                qb[i][j] = sbb[i][j] - sb[i][j] * sb[i][j] / n;
+                qab[i][j] = sab[i][j] - sa[i][j] * sb[i][j] / n;
+                const bool inside = x0 + ty * 4 + i < args.na && y0 + tx * 4 + j < args.nb;
+                ill = ill || (inside && !(qa[i][j] > saa[i][j] * 0.0625f && qb[i][j] > sbb[i][j] * 0.0625f));
             }
+        if (__syncthreads_or(ill) == 0) goto finish;
     }
 
+    // ---- two passes (tiles the one pass does not settle) -------------------
+    {
+        float s1[4][4], s2[4][4];
+    #pragma unroll
+        for (int i = 0; i < 4; ++i)
+    #pragma unroll
+            for (int j = 0; j < 4; ++j) cnt[i][j] = s1[i][j] = s2[i][j] = 0.0f;
+
+        // pass 1: count, sum x, sum y over the pair's samples.  A chunk without
+        // Inf uses products with 0/1 masks (x * 0 = 0 for finite x): one fma per
+        // sum instead of a select and an add, with the same sums bit for bit; a
+        // chunk with an Inf selects.
+        gload(0);
+        for (int64_t kb = 0; kb < dim; kb += WBK) {
+            __syncthreads();
+            stage();
+            bool inf_here = false;
+    #pragma unroll
+            for (int q = 0; q < 8; ++q) inf_here = inf_here || __builtin_isinf(ra[q]) || __builtin_isinf(rb[q]);
+            const bool any_inf = __syncthreads_or(inf_here) != 0;
+            if (kb + WBK < dim) gload(kb + WBK);
+            float p1[4][4], p2[4][4];
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (int j = 0; j < 4; ++j) p1[i][j] = p2[i][j] = 0.0f;
+            if (!any_inf) {
+    #pragma unroll 4
+                for (int kc = 0; kc < WBK; ++kc) {
+                    const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
+                    const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
+                    const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+                    const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+                    float am[4], a0[4], bm[4], b0[4];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        am[i] = av[i] == av[i] ? 1.0f : 0.0f;
+                        a0[i] = av[i] == av[i] ? av[i] : 0.0f;
+                        bm[i] = bv[i] == bv[i] ? 1.0f : 0.0f;
+                        b0[i] = bv[i] == bv[i] ? bv[i] : 0.0f;
+                    }
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            cnt[i][j] = fmaf(am[i], bm[j], cnt[i][j]);
+                            p1[i][j] = fmaf(a0[i], bm[j], p1[i][j]);
+                            p2[i][j] = fmaf(b0[j], am[i], p2[i][j]);
+                        }
+                }
+            } else {
+    #pragma unroll 4
+                for (int kc = 0; kc < WBK; ++kc) {
+                    const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
+                    const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
+                    const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+                    const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
+                            cnt[i][j] += m ? 1.0f : 0.0f;
+                            p1[i][j] += m ? av[i] : 0.0f;
+                            p2[i][j] += m ? bv[j] : 0.0f;
+                        }
+                }
+            }
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    s1[i][j] += p1[i][j];
+                    s2[i][j] += p2[i][j];
+                }
+        }
+        // the pair's means (0 / 0 = NaN for a pair without samples: its result
+        // is set below)
+    #pragma unroll
+        for (int i = 0; i < 4; ++i)
+    #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                s1[i][j] = s1[i][j] / cnt[i][j];
+                s2[i][j] = s2[i][j] / cnt[i][j];
+            }
+
+        // pass 2: sum dx^2, sum dy^2, sum dx dy
+    #pragma unroll
+        for (int i = 0; i < 4; ++i)
+    #pragma unroll
+            for (int j = 0; j < 4; ++j) qa[i][j] = qb[i][j] = qab[i][j] = 0.0f;
+        gload(0);
+        for (int64_t kb = 0; kb < dim; kb += WBK) {
+            __syncthreads();
+            stage();
+            __syncthreads();
+            if (kb + WBK < dim) gload(kb + WBK);
+            float pa2[4][4], pb2[4][4], pab[4][4];
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (int j = 0; j < 4; ++j) pa2[i][j] = pb2[i][j] = pab[i][j] = 0.0f;
+    #pragma unroll 2
+            for (int kc = 0; kc < WBK; ++kc) {
+                const float4 a4 = *reinterpret_cast<const float4 *>(&lds[0][kc][ty * 4]);
+                const float4 b4 = *reinterpret_cast<const float4 *>(&lds[1][kc][tx * 4]);
+                const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+    #pragma unroll
+                for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bool m = (av[i] == av[i]) && (bv[j] == bv[j]);
+                        const float d = m ? av[i] - s1[i][j] : 0.0f;
+                        const float e = m ? bv[j] - s2[i][j] : 0.0f;
+                        pa2[i][j] = fmaf(d, d, pa2[i][j]);
+                        pb2[i][j] = fmaf(e, e, pb2[i][j]);
+                        pab[i][j] = fmaf(d, e, pab[i][j]);
+                    }
+            }
+    #pragma unroll
+            for (int i = 0; i < 4; ++i)
+    #pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    qa[i][j] += pa2[i][j];
+                    qb[i][j] += pb2[i][j];
+                    qab[i][j] += pab[i][j];
+                }
+        }
+    }
+
+finish:
     // epilogue: no samples gives the reference's 0 / (0 - 1) = -0
     const bool mirror = SYM && (I != J);
     float rv[4][4], tv[4][4];
