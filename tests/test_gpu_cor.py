@@ -186,6 +186,38 @@ def test_gpucor_pairwise_special_cases():
     _check(res, x, y, "pairwise.complete.obs")
 
 
+@pytest.mark.parametrize("self_case", [False, True])
+def test_gpucor_pairwise_offsets_and_drift(self_case):
+    """The one-pass form shifts each variable by its mean over the first 32
+    samples.  Variables here sit far from zero (N(1000, 1)).  Some drift by
+    500 after the first 32 samples: the shift is off by most of the drift,
+    and the moments cancel by about 3 bits, within the one pass.  In others
+    the only present value among the first 32 is an outlier of 1e4, so the
+    shift is off by 9000 standard deviations: the moments cancel completely
+    and those tiles take the two passes.  Some have every one of the first
+    32 samples missing (shift 0).  Several 64 x 64 tiles, so one-pass and
+    two-pass tiles run side by side."""
+    rng = np.random.default_rng(91 + self_case)
+    size, nx, ny = 300, 150, 140
+    x = _data(rng, size, nx, shift=1000.0)
+    y = _data(rng, size, ny, shift=-250.0, scale=0.5)
+    x[32:, 20:30] += 500.0                  # drift in the first tile of x
+    y[32:, 5:9] -= 800.0                    # drift in the first tile of y
+    x[:32, 70:74] = np.nan                  # an outlier as the shift: the
+    x[3, 70:74] = 1.0e4                     # second tile of x is ill-conditioned
+    x[:32, 130:134] = np.nan                # first chunk all missing (shift 0):
+    y[:32, 130] = np.nan                    # ill-conditioned as well
+    x[rng.random(x.shape) < 0.1] = np.nan
+    y[rng.random(y.shape) < 0.1] = np.nan
+    print("case gpuCor pairwise offsets/drift self=%s" % self_case, flush=True)
+    if self_case:
+        res = gt.gpuCor(x, use="pairwise.complete.obs")
+        _check(res, x, None, "pairwise.complete.obs")
+        np.testing.assert_array_equal(res["coefficients"], res["coefficients"].T)
+    else:
+        _check(gt.gpuCor(x, y, use="pairwise.complete.obs"), x, y, "pairwise.complete.obs")
+
+
 def test_gpucor_zero_samples():
     print("case gpuCor zero samples", flush=True)
     res = gt.gpuCor(np.zeros((0, 3), np.float32), np.zeros((0, 2), np.float32))
