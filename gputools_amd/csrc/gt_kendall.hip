@@ -17,7 +17,9 @@
 // A wave compares x_i with 64 consecutive x_j at once (the loads of up to 32
 // such steps issued together) and __ballot turns the comparisons into bit
 // words, which v_writelane collects across lanes for coalesced 256-byte
-// stores.  The sample pairs of row i take
+// stores.  In a variable without NaN every sample pair is exactly one of P,
+// N and E, so only P and E are compared and N = ~(P | E) within the row.
+// The sample pairs of row i take
 // ceil((n - 1 - i) / 32) words (the tail bits are zero), so
 //   concordant(x, y) = sum over words of popc(Px & Py) + popc(Nx & Ny)
 //                      + popc(Ex & Ey)
@@ -77,6 +79,18 @@ __device__ __forceinline__ void put_step(uint32_t &wp, uint32_t &wn, uint32_t &w
                    "i"(L), "i"(L + 1));
 }
 
+// One plane's two words of one step into lanes L and L + 1 (the same wait
+// states as put_step).
+template <int L>
+__device__ __forceinline__ void put_plane(uint32_t &w, unsigned long long bits)
+{
+    asm volatile("s_nop 4\n\t"
+                 "v_writelane_b32 %0, %1, %3\n\t"
+                 "v_writelane_b32 %0, %2, %4"
+                 : "+v"(w)
+                 : "s"((uint32_t)bits), "s"((uint32_t)(bits >> 32)), "i"(L), "i"(L + 1));
+}
+
 // f(std::integral_constant<int, U>) for U = 0 .. N - 1, in order.
 template <int U, int N, typename F>
 __device__ __forceinline__ void static_for(F &&f)
@@ -95,8 +109,8 @@ __device__ __forceinline__ void static_for(F &&f)
 // words_upto(n - 1) - words_upto(n - 1 - i).  Words past the last row are
 // zero (cleared before this pass).  flags[v] (cleared before) gets bit 0 when
 // any sample pair of v is tied (its E plane has a set bit) and bit 1 when v
-// holds a NaN (every x_j with j < n - 1 is the x_i of a row; x_{n-1} is
-// checked on its own).
+// holds a NaN (a block-wide vote over all n values).
+template <bool LX>
 __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__restrict__ x,
                                                              int64_t nvar, int64_t n, int64_t ldx,
                                                              uint32_t *__restrict__ planes,
@@ -115,11 +129,30 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
     const int rs = (int)(blockIdx.x % rsplit);
     const float *xv = x + v * ldx;
     uint32_t *const pv = planes + v * wpad;
-    bool tied = false, has_nan = false;
+    // Does the variable hold a NaN?  Without one, every sample pair in a row
+    // is exactly one of P, N and E, so the N word is ~(P | E) within the row
+    // and needs no compare of its own; the E words need lane writes only in
+    // steps with a tie.  That takes a step from 3 compares and 6 lane writes
+    // to 2 and 2.
+    // LX: the variable's n values are staged in LDS (the same pass), followed
+    // by 2048 NaNs, and the rows read them from there instead of through the
+    // vector memory path.  A group reads all 32 steps, up to x_j with
+    // j = n - 1 + 63 + 31 * 64, so the NaN tail stands in for the bounds and
+    // step tests (a NaN compares false).
+    extern __shared__ float xs[];
+    bool nan_here = false;
+    for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
+        const float val = xv[t];
+        nan_here = nan_here || val != val;
+        if (LX) xs[t] = val;
+    }
+    if (LX)
+        for (int t = threadIdx.x; t < 2048; t += blockDim.x) xs[n + t] = qnan;
+    const bool has_nan = __syncthreads_or(nan_here) != 0;   // (and the LDS copy is complete)
+    bool tied = false;
     for (int64_t i = 4 * rs + wave; i < n - 1; i += 4 * (int64_t)rsplit) {
         const int64_t m = n - 1 - i;
-        const float xi = xv[i];
-        has_nan = has_nan || xi != xi;
+        const float xi = LX ? xs[i] : xv[i];
         const float *xr = xv + i + 1;                    // x_j, j = i + 1 + q
         uint32_t *const row = pv + (wtot - words_upto(m));
         const int64_t wr = (m + 31) / 32;
@@ -128,30 +161,64 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
         // u (a compile-time lane, so each word is one v_writelane), and the
         // group's 64 words per plane go out in one store per plane
         for (int64_t g0 = 0; g0 < steps; g0 += 32) {
-            const int nst = (int)((steps - g0) < 32 ? (steps - g0) : 32);
+            int nst = (int)((steps - g0) < 32 ? (steps - g0) : 32);
+            // a 32-bit scalar: the step tests below stay scalar compares
+            // (the compiler otherwise tests the 64-bit value on the VALU)
+            asm("" : "+s"(nst));
             float xj[32];
-#pragma unroll
-            for (int u = 0; u < 32; ++u) {
-                if (u >= nst) break;
-                const int64_t q = (g0 + u) * 64 + lane;
-                xj[u] = q < m ? xr[q] : qnan;
+            // every xj[u] a register of its own (compile-time u): a loop with
+            // a run-time exit would index xj dynamically and wait on each load
+            if (LX) {
+                // n <= 8192: 32-bit offsets; step u is an immediate offset
+                const float *xl = xs + (int)(i + 1 + g0 * 64) + lane;
+                static_for<0, 32>([&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    xj[u] = xl[u * 64];
+                });
+            } else {
+                static_for<0, 32>([&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    if (u < nst) {
+                        const int64_t q = (g0 + u) * 64 + lane;
+                        xj[u] = q < m ? xr[q] : qnan;
+                    }
+                });
             }
             uint32_t wp = 0, wn = 0, we = 0;
             bool group_tied = false;
-            static_for<0, 32>([&](auto uc) {
-                constexpr int u = decltype(uc)::value;
-                if (u < nst) {
-                    const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
-                    const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj[u] < xi);
-                    const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
-                    put_step<2 * u>(wp, wn, we, gt, lt, eq);
-                    group_tied = group_tied || eq != 0ull;
-                }
-            });
+            const int64_t w = 2 * g0 + lane;
+            if (has_nan) {
+                static_for<0, 32>([&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    if (u < nst) {
+                        const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
+                        const unsigned long long lt = __builtin_amdgcn_ballot_w64(xj[u] < xi);
+                        const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
+                        put_step<2 * u>(wp, wn, we, gt, lt, eq);
+                        group_tied = group_tied || eq != 0ull;
+                    }
+                });
+            } else {
+                static_for<0, 32>([&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    if (u < nst) {
+                        const unsigned long long gt = __builtin_amdgcn_ballot_w64(xj[u] > xi);
+                        const unsigned long long eq = __builtin_amdgcn_ballot_w64(xj[u] == xi);
+                        put_plane<2 * u>(wp, gt);
+                        if (eq != 0ull) {
+                            put_plane<2 * u>(we, eq);
+                            group_tied = true;
+                        }
+                    }
+                });
+                // the sample pairs of word w are q = 32 w .. 32 w + 31 < m
+                const int64_t rem = m - 32 * w;
+                const uint32_t valid = rem >= 32 ? 0xffffffffu : rem > 0 ? (1u << rem) - 1u : 0u;
+                wn = ~(wp | we) & valid;
+            }
             // lanes at or past `used` hold no word of this group; w < wr
             // drops the odd last word of the row (it belongs to the next row)
             const int used = 2 * nst;
-            const int64_t w = 2 * g0 + lane;
             if (lane < used && w < wr) {
                 row[w] = wp;
                 row[pstride + w] = wn;
@@ -161,8 +228,6 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
             tied = tied || group_tied;
         }
     }
-    // a tie anywhere in the variable: its E plane is not all zero
-    if (rs == 0 && wave == 0 && n >= 1 && xv[n - 1] != xv[n - 1]) has_nan = true;
     const uint32_t bits = (tied ? 1u : 0u) | (has_nan ? 2u : 0u);
     if (lane == 0 && bits) atomicOr(ties + v, bits);
 }
@@ -485,11 +550,15 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
             const int64_t nv = na > nb ? na : nb;
             if (nv * rs64 > 0x7fffffff) rs64 = 0x7fffffff / nv;
             const int rsplit = (int)(rs64 > 0 ? rs64 : 1);
-            hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)(na * rsplit)), dim3(256), 0, stream,
+            // up to 8192 samples (32 KB + an 8 KB tail) the rows read the variable from LDS
+            const bool lx = n <= 8192;
+            const unsigned lds = lx ? (unsigned)((n + 2048) * 4) : 0u;
+            auto planes_fn = lx ? kendall_planes_kernel<true> : kendall_planes_kernel<false>;
+            hipLaunchKernelGGL(planes_fn, dim3((unsigned)(na * rsplit)), dim3(256), lds, stream,
                                a, na, n, lda, pa, wpad, ta, rsplit);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if (!same) {
-                hipLaunchKernelGGL(kendall_planes_kernel, dim3((unsigned)(nb * rsplit)), dim3(256), 0,
+                hipLaunchKernelGGL(planes_fn, dim3((unsigned)(nb * rsplit)), dim3(256), lds,
                                    stream, b, nb, n, ldb, pb, wpad, tb, rsplit);
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }

@@ -123,3 +123,22 @@ def test_kendall_nan_and_clean_variables_mixed():
     torch.cuda.synchronize()
     want_self, _ = orc.kendall(a, a, nthreads=8)
     assert_same(self_tau.cpu().numpy(), want_self)
+
+
+def test_kendall_long_variables_global_rows():
+    """8200 samples: above 8192 the planes pass reads each row's values
+    through the vector memory path instead of an LDS copy of the variable.
+    Continuous, tied and NaN-holding variables side by side."""
+    rng = np.random.default_rng(31)
+    n = 8200
+    a = rng.standard_normal((4, n)).astype(np.float32)
+    b = rng.standard_normal((3, n)).astype(np.float32)
+    a[1] = rng.integers(0, 5, n)          # ties
+    a[2, 4000] = np.nan
+    b[0] = a[0] + rng.standard_normal(n).astype(np.float32) * 0.3
+    b[2, n - 1] = np.nan                  # the last sample is no row's x_i
+    print("case kendall n=8200 global rows", flush=True)
+    tau = gt.kendall(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    torch.cuda.synchronize()
+    want, _ = orc.kendall(a, b, nthreads=8)
+    assert_same(tau.cpu().numpy(), want)
