@@ -37,7 +37,7 @@ PEAK_TFLOPS = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp32": 157.3}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp16x3", "fp32"])
     ap.add_argument("--n", type=int, default=N_POINTS)
@@ -65,8 +65,9 @@ def parse():
 
 
 def time_path(lib, x, out, ws, ws_bytes, precision, steps, warmup, torch):
-    """Median-free plain timing: W untimed steps, then K steps between two HIP
-    events recorded on the stream the kernels are launched on."""
+    """W untimed steps, then K steps between two HIP events recorded on the
+    stream the kernels are launched on (the mean per step).  An event between
+    consecutive steps also gives each step's time, for the median."""
     from gputools_amd import _lib
     n, k = x.shape
     stream = torch.cuda.current_stream()
@@ -83,14 +84,14 @@ def time_path(lib, x, out, ws, ws_bytes, precision, steps, warmup, torch):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(steps):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
         step()
-    stop.record(stream)
+        ev[i + 1].record(stream)
     torch.cuda.synchronize()
-    return start.elapsed_time(stop) / steps
+    per_step = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return ev[0].elapsed_time(ev[steps]) / steps, float(np.median(per_step))
 
 
 def sampled_pairs(n, rows):
@@ -351,8 +352,8 @@ def main():
         steps = args.steps if precision == args.precision else (args.fp32_steps or args.steps)
         print("bench: timing %s, %d warmup + %d steps" % (precision, args.warmup, steps),
               file=sys.stderr, flush=True)
-        ms = time_path(lib, x, out, ws, ws_bytes, precision, steps, args.warmup, torch)
-        rec = {"ms_per_step": ms, "pairs_per_s": npairs / (ms * 1e-3)}
+        ms, med = time_path(lib, x, out, ws, ws_bytes, precision, steps, args.warmup, torch)
+        rec = {"ms_per_step": ms, "median_ms": med, "pairs_per_s": npairs / (ms * 1e-3)}
         if not args.no_cpu_baseline:
             rng = np.random.default_rng(12345)
             rows = rng.choice(n, size=min(args.sample_rows, n), replace=False)
@@ -406,6 +407,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "median_ms_per_step": round(head["median_ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -419,8 +421,10 @@ def main():
     if args.precision != "fp32" and "fp32" in results:
         f = results["fp32"]
         line["fp32_plain"] = {"ms_per_step": round(f["ms_per_step"], 4),
+                              "median_ms_per_step": round(f["median_ms"], 4),
                               "pairs_per_s": round(f["pairs_per_s"], 1)}
         line["speedup_vs_fp32"] = round(f["ms_per_step"] / ms, 3)
+        line["speedup_vs_fp32_median"] = round(f["median_ms"] / head["median_ms"], 3)
     for other, rec in results.items():
         if other not in (args.precision, "fp32"):
             line.setdefault("other_precisions", {})[other] = {
