@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--paths-only", action="store_true",
                     help="time only the selected secondary paths (no headline line; for "
                          "profiling them)")
+    ap.add_argument("--host-path", action="store_true",
+                    help="also time the host-buffer entry point (PCIe-inclusive; not `value`)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (reference kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -455,6 +457,32 @@ def main():
         line["max_rel_err"] = errs[args.precision]
         line["max_rel_err_by_path"] = {p: float("%.3g" % e) for p, e in errs.items()}
         line["error_sample_pairs"] = int(ref.size)
+
+    if args.host_path:
+        # The host-buffer boundary (gt_dist_host, the counterpart of distance()):
+        # H2D of the points, the kernels, D2H of the packed result into pageable
+        # host memory, device allocations included.  Never `value`.
+        from gputools_amd import _lib
+        out_h = np.empty(npairs, dtype=np.float32)
+        prec = _lib.PRECISIONS[args.precision]
+
+        def host_call():
+            s = lib.gt_dist_host(xh.ctypes.data, n, k, _lib.EUCLIDEAN, 2.0, prec,
+                                 _lib.LAYOUT_PACKED, out_h.ctypes.data)
+            if s != 0:
+                raise RuntimeError(_lib.last_error())
+
+        print("bench: host-buffer path, 1 warm-up + 2 timed calls", file=sys.stderr, flush=True)
+        host_call()                          # also first-touches the output pages
+        t0 = time.perf_counter()
+        for _ in range(2):
+            host_call()
+        hms = (time.perf_counter() - t0) / 2 * 1e3
+        line["host_path"] = {"ms": round(hms, 2), "pairs_per_s": round(npairs / (hms * 1e-3), 1),
+                             "what": "gt_dist_host: H2D of X, kernels, D2H of the %.2f GB packed "
+                                     "result to pageable host memory, allocations included"
+                                     % (4.0 * npairs / 1e9)}
+        del out_h
 
     if not args.no_paths:
         print("bench: secondary paths", file=sys.stderr, flush=True)
