@@ -30,8 +30,8 @@
 // tile without NaN, a chunk without E bits has exactly one of the P and N
 // bits of every sample pair set, so
 //   concordant(x, y) = sum over words of popc(~(Px ^ Py)) - (bits past row ends),
-// 2 operations per 32 sample pairs (the bits past row ends are counted once
-// per chunk from the first variable's P | N | E).  The count
+// 2 operations per 32 sample pairs (the bits past row ends depend on the
+// layout alone: a small pass tabulates them per chunk).  The count
 // kernel tiles 128 x 128 pairs of variables over 256 lanes (8 x 8 each),
 // stages word chunks of both sides through LDS and splits the word range over
 // workgroups, adding its counts with 64-bit atomics.  A last pass applies the
@@ -240,6 +240,29 @@ constexpr int KKW = 8;                   // words per LDS chunk (256 sample pair
 constexpr int KTHREADS = 256;            // 16 x 16 lanes, 8 x 8 pairs each
 constexpr int KUNITS = 2 * 3 * KBM * 2 / KTHREADS;  // 16-byte staging units per lane
 
+// Bits past row ends per chunk of KKW words, from the layout alone: the last
+// word of row i (m = n - 1 - i sample pairs) holds m % 32 of them when that is
+// not 0, and the words after the last row (up to wpad) are past the end
+// entirely.  pe (cleared before) gets one count per chunk.
+__global__ __launch_bounds__(256) void kendall_pastend_kernel(int64_t n, int64_t wpad,
+                                                              uint32_t *__restrict__ pe)
+{
+    const int64_t wtot = words_upto(n - 1);
+    const int64_t rows = n - 1, tail = wpad - wtot;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < rows + tail;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        if (t < rows) {
+            const int64_t m = n - 1 - t;
+            if (m % 32 != 0) {
+                const int64_t last = wtot - words_upto(m) + (m + 31) / 32 - 1;
+                atomicAdd(pe + last / KKW, 32u - (uint32_t)(m % 32));
+            }
+        } else {
+            atomicAdd(pe + (wtot + (t - rows)) / KKW, 32u);
+        }
+    }
+}
+
 struct KendallArgs {
     const uint32_t *pa, *pb;   // planes of A (na variables) and B (nb variables)
     int64_t na, nb, wpad;
@@ -247,6 +270,7 @@ struct KendallArgs {
     int64_t ldc;
     int64_t tiles_a, tiles_b, ntiles;
     int64_t split_w;           // words per workgroup (multiple of KKW)
+    const uint32_t *pe;        // per chunk of KKW words: bits past row ends
     const uint32_t *ties_a, *ties_b;   // per variable: bit 0 a tie (E bit set), bit 1 a NaN
     int64_t id_offset, ids_this;
 };
@@ -435,12 +459,8 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         }
         if (w + KKW < w_end) gload(w + KKW);
         if (!any_e && !nan_tile) {
-            // the chunk's bits past row ends, from the first variable (with
-            // no NaN, P | N | E marks every sample pair)
-#pragma unroll
-            for (int kw = 0; kw < KKW; ++kw)
-                past_end += 32u - (uint32_t)__builtin_popcount(lds[0][0][kw][0] | lds[0][1][kw][0] |
-                                                               lds[0][2][kw][0]);
+            // the chunk's bits past row ends (the same for every variable)
+            past_end += args.pe[w / KKW];
             count_chunk_xnor(lds, tx, ty, cnt);
         } else if (any_e && use_e) {
             count_chunk<3>(lds, tx, ty, cnt);
@@ -508,7 +528,8 @@ size_t kendall_workspace_bytes(int64_t na, int64_t nb, bool same, int64_t n)
     const size_t planes = align256(3 * (size_t)na * wpad * 4) +
                           (same ? 0 : align256(3 * (size_t)nb * wpad * 4));
     const size_t ties = align256((size_t)na * 4) + (same ? 0 : align256((size_t)nb * 4));
-    return planes + ties + align256((size_t)na * (size_t)nb * 8);
+    const size_t pe = align256(wpad / KKW * 4);
+    return planes + ties + pe + align256((size_t)na * (size_t)nb * 8);
 }
 
 hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
@@ -519,7 +540,7 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
     hipError_t e;
     const int64_t wpad = padded_words(n);
     char *ws = static_cast<char *>(workspace);
-    uint32_t *pa = nullptr, *pb = nullptr, *ta = nullptr, *tb = nullptr;
+    uint32_t *pa = nullptr, *pb = nullptr, *ta = nullptr, *tb = nullptr, *pe = nullptr;
     unsigned long long *count = nullptr;
     if (n >= 2) {
         pa = reinterpret_cast<uint32_t *>(ws);
@@ -536,8 +557,10 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
             tb = reinterpret_cast<uint32_t *>(ws);
             ws += align256((size_t)nb * 4);
         }
+        pe = reinterpret_cast<uint32_t *>(ws);
+        ws += align256((size_t)(wpad / KKW) * 4);
         count = reinterpret_cast<unsigned long long *>(ws);
-        // zero planes (tail words), tie flags and counts with a fill kernel, not a
+        // zero planes (tail words), tie flags, past-end counts and counts with a fill kernel, not a
         // memset node, so that a captured graph replays correctly
         const size_t zero_floats = (size_t)(reinterpret_cast<char *>(count) - static_cast<char *>(workspace)) / 4 +
                                    (size_t)na * (size_t)nb * 2;
@@ -563,7 +586,14 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         }
+        {
+            const int64_t items = (n - 1) + (wpad - words_upto(n - 1));
+            hipLaunchKernelGGL(kendall_pastend_kernel, dim3(grid_for(items, 256)), dim3(256), 0,
+                               stream, n, wpad, pe);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         KendallArgs args;
+        args.pe = pe;
         args.pa = pa;
         args.pb = pb;
         args.na = na;
