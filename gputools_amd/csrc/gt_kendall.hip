@@ -318,7 +318,10 @@ __device__ __forceinline__ void count_chunk(const uint32_t (&lds)[2][3][KKW][KBM
 __device__ __forceinline__ void count_chunk_xnor(const uint32_t (&lds)[2][3][KKW][KBM], int tx,
                                                  int ty, uint32_t (&cnt)[8][8])
 {
-#pragma unroll 2
+    // fully unrolled: a rolled loop here is a loop of its own, and the
+    // compiler then copies the 64 counts into and out of its registers on
+    // every chunk
+#pragma unroll
     for (int kw = 0; kw < KKW; ++kw) {
         const uint4 a0 = *reinterpret_cast<const uint4 *>(&lds[0][0][kw][ty * 4]);
         const uint4 a1 = *reinterpret_cast<const uint4 *>(&lds[0][0][kw][64 + ty * 4]);
