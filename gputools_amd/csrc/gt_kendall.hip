@@ -399,11 +399,10 @@ __global__ __launch_bounds__(KTHREADS) void kendall_count_kernel(KendallArgs arg
         const int64_t gv = (side ? y0 : x0) + v;
         // E words are loaded for a side with a tied variable: E & E needs
         // both sides, the XNOR form's check needs either.  N words are loaded
-        // ahead only in a tile with a NaN, and for the first variable (its
-        // P | N | E marks the sample pairs); other chunks that need them
-        // load them when their E bits show up.
+        // ahead only in a tile with a NaN; other chunks that need them load
+        // them when their E bits show up.
         inr[q] = gv < nside;
-        ok[q] = inr[q] && (plane == 0 || (plane == 1 && (nan_tile || (side == 0 && v == 0))) ||
+        ok[q] = inr[q] && (plane == 0 || (plane == 1 && nan_tile) ||
                            (plane == 2 && (side ? tie_y : tie_x)));
         src[q] = (side ? args.pb : args.pa) + ((int64_t)plane * nside + (inr[q] ? gv : 0)) * args.wpad +
                  half * 4;
