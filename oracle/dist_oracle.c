@@ -1,9 +1,9 @@
 /*
  * oracle/dist_oracle.c -- FP64 CPU restatement of the pairwise-distance path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
- * gputools_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline leg
- * of bench.py may load it.  The product never links it and never falls back
+ * TEST INFRASTRUCTURE ONLY.  The HIP library in gputools_amd/ is tested
+ * against this file: only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product never links it and never falls back
  * to it.
  *
  * What it restates
