@@ -107,7 +107,8 @@ __device__ __forceinline__ void static_for(F &&f)
 // Plane p (0 = P, 1 = N, 2 = E) of variable v, word w at
 // planes[(p * nvar + v) * wpad + w]; row i starts at word
 // words_upto(n - 1) - words_upto(n - 1 - i).  Words past the last row are
-// zero (cleared before this pass).  flags[v] (cleared before) gets bit 0 when
+// zero (written by this pass in P and N; the E planes are cleared before it,
+// since E words are written only where a step has a tie).  flags[v] (cleared before) gets bit 0 when
 // any sample pair of v is tied (its E plane has a set bit) and bit 1 when v
 // holds a NaN (a block-wide vote over all n values).
 template <bool LX>
@@ -149,6 +150,12 @@ __global__ __launch_bounds__(256) void kendall_planes_kernel(const float *__rest
     if (LX)
         for (int t = threadIdx.x; t < 2048; t += blockDim.x) xs[n + t] = qnan;
     const bool has_nan = __syncthreads_or(nan_here) != 0;   // (and the LDS copy is complete)
+    // P and N are written in full, up to wpad: split 0 zeroes the words after
+    // the last row (fewer than KKW); the E planes are cleared before this pass
+    if (rs == 0 && threadIdx.x < wpad - wtot) {
+        pv[wtot + threadIdx.x] = 0u;
+        pv[pstride + wtot + threadIdx.x] = 0u;
+    }
     bool tied = false;
     for (int64_t i = 4 * rs + wave; i < n - 1; i += 4 * (int64_t)rsplit) {
         const int64_t m = n - 1 - i;
@@ -562,11 +569,22 @@ hipError_t launch_kendall(const float *a, int64_t na, int64_t lda, const float *
         pe = reinterpret_cast<uint32_t *>(ws);
         ws += align256((size_t)(wpad / KKW) * 4);
         count = reinterpret_cast<unsigned long long *>(ws);
-        // zero planes (tail words), tie flags, past-end counts and counts with a fill kernel, not a
-        // memset node, so that a captured graph replays correctly
-        const size_t zero_floats = (size_t)(reinterpret_cast<char *>(count) - static_cast<char *>(workspace)) / 4 +
-                                   (size_t)na * (size_t)nb * 2;
-        e = launch_fill(reinterpret_cast<float *>(workspace), (int64_t)zero_floats, 0.0f, stream);
+        // zero the E planes (written only where a step has a tie), tie flags,
+        // past-end counts and counts with fill kernels, not memset nodes, so
+        // that a captured graph replays correctly; the planes pass writes P
+        // and N in full
+        e = launch_fill(reinterpret_cast<float *>(pa + 2 * (size_t)na * (size_t)wpad),
+                        na * wpad, 0.0f, stream);
+        if (e != hipSuccess) return e;
+        if (!same) {
+            e = launch_fill(reinterpret_cast<float *>(pb + 2 * (size_t)nb * (size_t)wpad),
+                            nb * wpad, 0.0f, stream);
+            if (e != hipSuccess) return e;
+        }
+        const size_t zero_floats =
+            (size_t)(reinterpret_cast<char *>(count + (size_t)na * (size_t)nb) -
+                     reinterpret_cast<char *>(ta)) / 4;
+        e = launch_fill(reinterpret_cast<float *>(ta), (int64_t)zero_floats, 0.0f, stream);
         if (e != hipSuccess) return e;
         {
             // row splits per variable: about 16 rows per wave, within the
