@@ -320,7 +320,8 @@ int gt_ttest(const float *pairs, const float *corr, int64_t n, float *ts, void *
  * are counted exactly in integers; the quotient follows the reference's FP32
  * / FP64 steps (inst/cuda/kendall.cu:40-45).  b == NULL (or b == a with the
  * same shape) is the self case.  The workspace (gt_kendall_workspace_size)
- * holds three bit planes of n(n-1)/2 bits per variable and the counts. */
+ * holds three bit planes of n(n-1)/2 bits per variable, per-variable flags,
+ * a small per-chunk table of the bits past row ends, and the counts. */
 int gt_kendall_workspace_size(int64_t na, int64_t nb, int64_t n, int same, size_t *bytes);
 int gt_kendall(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb, int64_t ldb,
                int64_t n, double *tau, int64_t ldt, void *workspace, size_t workspace_bytes,
