@@ -83,6 +83,8 @@ SIGNATURES = {
     "gt_hclust_workspace_size": (c_int, [c_i64, ctypes.POINTER(c_size)]),
     "gt_hclust": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p, c_p, c_size,
                           c_p]),
+    "gt_hclust_state": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p, c_p,
+                                c_size, c_int, c_p]),
     "gt_hclust_host": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_float, c_p, c_p, c_p]),
     "gt_hclust_format": (c_int, [c_i64, c_p, c_p, c_p, c_p]),
     "rpmcc": (None, [c_ip, c_fp, c_ip, c_fp, c_ip, c_ip, c_fp, c_fp, c_fp]),

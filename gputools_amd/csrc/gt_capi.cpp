@@ -169,7 +169,10 @@ int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, floa
 
 // Stream-ordered scratch: hipMallocAsync / hipFreeAsync on the caller's stream
 // (safe against concurrent calls on other streams).  Falls back to a
-// synchronous hipMalloc/hipFree pair where the pool is unavailable.
+// synchronous hipMalloc/hipFree pair only where the stream-ordered pool is
+// not supported, and never while the stream is being captured into a graph
+// (a synchronous allocation or synchronisation would invalidate the capture):
+// callers that capture pass their own workspace (the *_ws entry points).
 struct Scratch {
     void *ptr = nullptr;
     bool async = false;
@@ -186,6 +189,13 @@ struct Scratch {
         }
         (void)hipGetLastError();
         ptr = nullptr;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone)
+            return fail(GT_ERR_WORKSPACE,
+                        "stream-ordered workspace allocation failed during graph capture (%s); "
+                        "pass a workspace (gt_dist_ws)",
+                        hipGetErrorString(e));
+        if (e != hipErrorNotSupported) return hip_fail(e, "hipMallocAsync(workspace)");
         e = hipMalloc(&ptr, bytes);
         if (e != hipSuccess) {
             ptr = nullptr;
@@ -301,13 +311,14 @@ int check_hclust_args(int64_t n, int64_t ldd, int method)
 }
 
 int run_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta, int *sub,
-               int *sup, float *height, void *ws, hipStream_t stream)
+               int *sup, float *height, void *ws, hipStream_t stream,
+               int state = gt::HC_STATE_AUTO)
 {
     char *w = static_cast<char *>(ws);
     const size_t part = round_up((size_t)n * 4, 256);
     GT_HIP(gt::launch_hclust(d, n, ldd, method, lambda, beta, reinterpret_cast<float *>(w),
                              reinterpret_cast<float *>(w + part), reinterpret_cast<int *>(w + 2 * part),
-                             sub, sup, height, stream));
+                             sub, sup, height, state, stream));
     return GT_OK;
 }
 
@@ -571,6 +582,7 @@ int gt_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float
     return s != GT_OK ? s : r;
 }
 
+
 int gt_dist_cross(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
                   int64_t ldb, int64_t k, int method, float p, float *out, int64_t ldd,
                   void *stream)
@@ -687,6 +699,28 @@ int gt_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float 
                     hclust_ws_bytes(n), workspace_bytes);
     return run_hclust(d, n, ldd, method, lambda, beta, sub, sup, height, workspace,
                       static_cast<hipStream_t>(stream));
+}
+
+int gt_hclust_state(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                    int *sub, int *sup, float *height, void *workspace, size_t workspace_bytes,
+                    int state, void *stream)
+{
+    int s = check_hclust_args(n, ldd, method);
+    if (s != GT_OK) return s;
+    if (state != GT_HC_STATE_AUTO && state != GT_HC_STATE_LDS && state != GT_HC_STATE_GLOBAL)
+        return fail(GT_ERR_INVALID, "unknown clustering state placement %d", state);
+    if (state == GT_HC_STATE_LDS && n > gt::HC_LDS_MAX_N)
+        return fail(GT_ERR_INVALID, "the LDS placement holds at most %lld points (n=%lld)",
+                    (long long)gt::HC_LDS_MAX_N, (long long)n);
+    if (n < 2) return GT_OK;
+    if (!d || !sub || !sup || !height) return fail(GT_ERR_INVALID, "null pointer");
+    if (workspace == nullptr || workspace_bytes < hclust_ws_bytes(n))
+        return fail(GT_ERR_WORKSPACE, "clustering workspace of %zu bytes needed (got %zu)",
+                    hclust_ws_bytes(n), workspace_bytes);
+    static_assert((int)GT_HC_STATE_LDS == (int)gt::HC_STATE_LDS &&
+                  (int)GT_HC_STATE_GLOBAL == (int)gt::HC_STATE_GLOBAL, "state values");
+    return run_hclust(d, n, ldd, method, lambda, beta, sub, sup, height, workspace,
+                      static_cast<hipStream_t>(stream), state);
 }
 
 int gt_hclust_host(const float *d_host, int64_t n, int64_t ldd, int method, float lambda,

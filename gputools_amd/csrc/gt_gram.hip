@@ -37,12 +37,11 @@
 namespace gt {
 namespace {
 
-constexpr int GBM = 256;        // tile side (points)
+constexpr int GBM = 256;        // B-side tile rows (i); the A side (j) has WA * 64
 constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
 constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
-constexpr int GTHREADS = 1024;  // 16 waves in a 4 (A) x 4 (B) grid, each 64 x 64 pairs
-constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
+constexpr int GSUPER = 4;       // super-tile side in 256-row tiles (L2 locality)
 constexpr int FBK = 32;         // features per fix-up chunk
 constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
 constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
@@ -50,23 +49,36 @@ constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
 // plus one residual correction (sqrt_rn), which rounds correctly for normal
 // inputs; d^2 below 2^-100 goes to the exact pass instead.
 constexpr float GRAM_SQRT_MIN = 0x1p-100f;
-constexpr int FPER = FCAP / GTHREADS;
-constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
-
-constexpr int TERM_BYTES = GBM * GROWB;      // one split term of one side, one chunk (8 KB)
-constexpr int FIX_BYTES = 2 * GBM * FLD * 4; // fix-up staging, both sides
 constexpr int LIST_BYTES = FCAP * 4;
 
+// Tile shape: WA x 4 waves, each 64 x 64 pairs, so WA * 64 A-side rows (j)
+// against 256 B-side rows (i).  WA = 4: one 1024-thread workgroup per CU and
+// a three-stage ring.  WA = 2: 512 threads, a 128 x 256 half of a 256 x 256
+// tile, two workgroups per CU (each workgroup's epilogue and exact pass
+// overlap the other's MFMA loop), two stages for three split terms.
 // LDS: [main-loop split terms, NSTAGE stages | (aliased) fix-up staging +
-// pair list] [norms, scales, scan]
-template <int NT>
-struct GramLds {
-    static constexpr int STAGE = 2 * NT * TERM_BYTES;
-    static constexpr int NSTAGE = 3;
+// pair list] [norms A, norms B, inverse scales A, B, scan]
+template <int NT, int WA>
+struct GramCfg {
+    static constexpr int BA = WA * 64;
+    static constexpr int BB = GBM;
+    static constexpr int NW = WA * 4;
+    static constexpr int THREADS = NW * 64;
+    static constexpr int HALVES = 4 / WA;           // A-side parts of a 256 x 256 tile
+    static constexpr int NG = (BA + BB) / 32;       // 32-row load groups
+    static constexpr int TERMA = BA * GROWB;        // one term, A side, one chunk
+    static constexpr int TERMB = BB * GROWB;
+    static constexpr int STAGE = NT * (TERMA + TERMB);
+    static constexpr int NSTAGE = (WA == 4 || NT == 2) ? 3 : 2;
     static constexpr int MAIN = NSTAGE * STAGE;
-    static constexpr int FIX = FIX_BYTES + LIST_BYTES;
+    static constexpr int FIXB = (BA + BB) * FLD * 4;
+    static constexpr int FIX = FIXB + LIST_BYTES;
     static constexpr int REGION = MAIN > FIX ? MAIN : FIX;
-    static constexpr int BYTES = REGION + 4 * GBM * 4 + 16 * 4;
+    static constexpr int BYTES = REGION + 2 * (BA + BB) * 4 + 16 * 4;
+    static constexpr int FUNITS = (BA + BB) * FBK / 4 / THREADS;  // float4 staging units per thread
+    static constexpr int FPER = FCAP / THREADS;
+    static_assert((BA + BB) * FBK / 4 % THREADS == 0, "staging units");
+    static_assert(FCAP % THREADS == 0, "list");
 };
 
 struct GramArgs {
@@ -83,12 +95,13 @@ struct GramArgs {
     float tau;
 };
 
-// Pre-pass: split every value once.  Term t of row r, feature c, at
-// terms[(t * n + r) * kp + c]; features k..kp-1 are zero.  One thread per 8
-// consecutive features (one 16 B store per term).
+// Pre-pass: split every centred value x - c once.  Term t of row r, feature
+// c, at terms[(t * n + r) * kp + c]; features k..kp-1 are zero.  One thread
+// per 8 consecutive features (one 16 B store per term).
 template <int SPLIT>
 __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x, int64_t n,
                                                     int64_t k, int64_t ldx, int64_t kp,
+                                                    const float *__restrict__ center,
                                                     const float *__restrict__ scales,
                                                     unsigned short *__restrict__ terms)
 {
@@ -102,7 +115,7 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
         unsigned short t[8][NT];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            float v = (c0 + c < k) ? x[r * ldx + c0 + c] : 0.0f;
+            float v = (c0 + c < k) ? x[r * ldx + c0 + c] - center[c0 + c] : 0.0f;
             if (SPLIT == SPLIT_FP16) v *= sc;
             Split<SPLIT>::split(v, t[c]);
         }
@@ -118,7 +131,8 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     }
 }
 
-// Exclusive scan of one int per thread over the GTHREADS-thread block.
+// Exclusive scan of one int per thread over a workgroup of NW waves.
+template <int NW>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -133,7 +147,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
     int before = 0;
     total = 0;
 #pragma unroll
-    for (int w = 0; w < GTHREADS / 64; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const int c = tmp[w];
         if (w < wave) before += c;
         total += c;
@@ -142,77 +156,106 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
     return before + incl - v;
 }
 
-template <int SPLIT, int LAYOUT, bool VEC4>
-__global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
+// Wait until at most N vector memory operations of this wave are outstanding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+template <int SPLIT, int LAYOUT, bool VEC4, int WA>
+__global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
+    GramArgs args)
 {
     typedef typename Split<SPLIT>::vec vec;
     constexpr int NT = Split<SPLIT>::NT;
-    typedef GramLds<NT> L;
-    __shared__ __attribute__((aligned(1024))) unsigned char smem[L::BYTES];
+    typedef GramCfg<NT, WA> C;
+    constexpr int BA = C::BA, BB = C::BB, THREADS = C::THREADS;
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[C::BYTES];
     float *fS = reinterpret_cast<float *>(smem);          // fix-up staging (aliases the terms)
-    unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
-    float *nA = reinterpret_cast<float *>(smem + L::REGION);  // row norms, A side (j)
-    float *nB = nA + GBM;                                 // B side (i)
-    float *iA = nB + GBM;                                 // inverse row scales (FP16 path)
-    float *iB = iA + GBM;
-    int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
+    unsigned int *list = reinterpret_cast<unsigned int *>(smem + C::FIXB);
+    float *nA = reinterpret_cast<float *>(smem + C::REGION);  // row norms, A side (j)
+    float *nB = nA + BA;                                  // B side (i)
+    float *iA = nB + BB;                                  // inverse row scales (FP16 path)
+    float *iB = iA + BA;
+    int *scan_tmp = reinterpret_cast<int *>(iB + BB);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave >> 2, wc = wave & 3;   // A rows wr*64.., B rows wc*64..
 
     // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
-    // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
+    // GSUPER tiles of 256 x 256, each cut into HALVES parts along j; tiles
+    // above the diagonal or past the edge exit at once.
     const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
-    const int64_t st = id / (GSUPER * GSUPER);
-    const int w = (int)(id % (GSUPER * GSUPER));
+    constexpr int PER_SUPER = GSUPER * GSUPER * C::HALVES;
+    const int64_t st = id / PER_SUPER;
+    const int w = (int)(id % PER_SUPER);
+    const int u = w / C::HALVES, h = w % C::HALVES;
     int SI, SJ;
     tri_tile(st, SI, SJ);
-    const int64_t I = (int64_t)SI * GSUPER + w / GSUPER;
-    const int64_t J = (int64_t)SJ * GSUPER + w % GSUPER;
+    const int64_t I = (int64_t)SI * GSUPER + u / GSUPER;
+    const int64_t J = (int64_t)SJ * GSUPER + u % GSUPER;
     if (I >= args.T || J > I) return;
-    const int64_t i0 = I * GBM, j0 = J * GBM;
+    const int64_t i0 = I * BB, j0 = J * GBM + (int64_t)h * BA;
     const int64_t n = args.n, k = args.k, kp = args.kp;
+    if (j0 >= n) return;
 
-    if (tid < GBM) {
+    if (tid < BA) {
         const int64_t r = j0 + tid;
         nA[tid] = r < n ? args.norms[r] : 0.0f;
         if (SPLIT == SPLIT_FP16) iA[tid] = r < n ? args.iscale[r] : 1.0f;
-    } else if (tid < 2 * GBM) {
-        const int64_t r = i0 + (tid - GBM);
-        nB[tid - GBM] = r < n ? args.norms[r] : 0.0f;
-        if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < n ? args.iscale[r] : 1.0f;
+    } else if (tid < BA + BB) {
+        const int64_t r = i0 + (tid - BA);
+        nB[tid - BA] = r < n ? args.norms[r] : 0.0f;
+        if (SPLIT == SPLIT_FP16) iB[tid - BA] = r < n ? args.iscale[r] : 1.0f;
     }
 
     // ---- main loop: direct global -> LDS loads, NSTAGE-deep, MFMA products -
-    // LDS chunk image: for stage g, side s (0 = A, rows j; 1 = B, rows i) and
-    // term t, 256 rows x 32 B; the two 16 B halves of row R are swapped when
+    // LDS chunk image: for stage g, side A (rows j) then side B (rows i), term
+    // t, rows x 32 B; the two 16 B halves of row R are swapped when
     // swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
     // 8 or 16 consecutive rows hit distinct banks without padding.
     // global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so
-    // each wave instruction fills 32 rows: waves 0-7 fill side A and waves
-    // 8-15 side B, wave w rows 32(w%8)..32(w%8)+31 of every term block of its
-    // side.  Rows past n are clamped to row n-1: they only feed pairs that are
-    // never stored.
+    // each wave instruction fills 32 rows of one term.  The (BA + BB) / 32 row
+    // groups go to the waves round robin (group q*NW + wave).  Rows past n are
+    // clamped to row n-1: they only feed pairs that are never stored.
     typedef __attribute__((address_space(3))) void lds_void_t;
     typedef const __attribute__((address_space(1))) void gbl_void_t;
     const int64_t term_stride = n * kp;
-    const int lside = wave >> 3, lrb = wave & 7;
-    const unsigned short *gsrc;
+    constexpr int GPW = (C::NG + C::NW - 1) / C::NW;   // row groups per wave, at most
+    const int ngw = (wave + C::NW < C::NG) ? 2 : 1;     // this wave's (GPW <= 2)
+    static_assert(GPW <= 2, "load groups");
+    const unsigned short *gsrc[GPW];
+    int ldst[GPW];          // LDS offset of the group's term-0 block within a stage
+    int tstride[GPW];       // bytes between the group's term blocks
     {
-        const int row = lrb * 32 + (lane >> 1);
-        const int half = (lane & 1) ^ (((row >> 2) ^ (row >> 3)) & 1);
-        const int64_t r = (lside ? i0 : j0) + row;
-        gsrc = args.terms + (r < n ? r : n - 1) * kp + half * 8;
+        const int rr = lane >> 1;
+        const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
+#pragma unroll
+        for (int q = 0; q < GPW; ++q) {
+            const int g = wave + q * C::NW;
+            const bool sideA = g < BA / 32;
+            const int grow = sideA ? g * 32 : (g - BA / 32) * 32;
+            const int64_t r = (sideA ? j0 : i0) + grow + rr;
+            gsrc[q] = args.terms + (r < n ? r : n - 1) * kp + half * 8;
+            ldst[q] = sideA ? grow * GROWB : NT * C::TERMA + grow * GROWB;
+            tstride[q] = sideA ? C::TERMA : C::TERMB;
+        }
     }
-    unsigned char *const lbase = smem + lrb * 1024;   // + block/stage offsets
     auto issue = [&](int chunk) {
-        const int g = chunk % L::NSTAGE;
+        const int g = chunk % C::NSTAGE;
         const int64_t kb = (int64_t)chunk * GBK;
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-            __builtin_amdgcn_global_load_lds(
-                (gbl_void_t *)(gsrc + t * term_stride + kb),
-                (lds_void_t *)(lbase + g * L::STAGE + (lside * NT + t) * TERM_BYTES), 16, 0, 0);
+        for (int q = 0; q < GPW; ++q) {
+            if (q < ngw) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    __builtin_amdgcn_global_load_lds(
+                        (gbl_void_t *)(gsrc[q] + t * term_stride + kb),
+                        (lds_void_t *)(smem + g * C::STAGE + ldst[q] + t * tstride[q]), 16, 0,
+                        0);
+            }
+        }
     };
     // operand read offset of this lane (rows differ by multiples of 32 between
     // blocks, which leaves swz unchanged)
@@ -230,39 +273,39 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     const int nchunks = (int)(kp / GBK);
     // prologue: chunks 0 .. NSTAGE-2 in flight
 #pragma unroll
-    for (int c = 0; c < L::NSTAGE - 1; ++c)
+    for (int c = 0; c < C::NSTAGE - 1; ++c)
         if (c < nchunks) issue(c);
     for (int c = 0; c < nchunks; ++c) {
-        // chunk c landed when at most the next chunk's NT loads per wave are
+        // chunk c landed when at most the later chunks already issued are
         // outstanding (vector memory loads complete in order)
-        if (c + 1 < nchunks) {
-            if (NT == 3)
-                asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+        if (C::NSTAGE == 3 && c + 1 < nchunks) {
+            if (ngw == 2)
+                wait_vmcnt<2 * NT>();
             else
-                asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+                wait_vmcnt<NT>();
         } else {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            wait_vmcnt<0>();
         }
         // every wave's part of chunk c is in LDS, and every wave finished
         // chunk c-1, whose stage the next issue overwrites.  A bare s_barrier:
         // __syncthreads() would also drain the later chunks' loads.
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (c + L::NSTAGE - 1 < nchunks) issue(c + L::NSTAGE - 1);
-        const unsigned char *sg = smem + (c % L::NSTAGE) * L::STAGE + roff;
+        if (c + C::NSTAGE - 1 < nchunks) issue(c + C::NSTAGE - 1);
+        const unsigned char *sg = smem + (c % C::NSTAGE) * C::STAGE + roff;
         vec av[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int a = 0; a < 2; ++a)
-                av[t][a] = *reinterpret_cast<const vec *>(sg + t * TERM_BYTES +
+                av[t][a] = *reinterpret_cast<const vec *>(sg + t * C::TERMA +
                                                           (wr * 64 + a * 32) * GROWB);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             vec bv[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t)
-                bv[t] = *reinterpret_cast<const vec *>(sg + (NT + t) * TERM_BYTES +
+                bv[t] = *reinterpret_cast<const vec *>(sg + NT * C::TERMA + t * C::TERMB +
                                                        (wc * 64 + b * 32) * GROWB);
             // hh' first, then the cross terms in decreasing magnitude
 #pragma unroll
@@ -285,7 +328,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     // 32x32 accumulator layout: element r of lane l is row 8(r/4) + 4(l/32) +
     // r%4 (A side, j) and column l%32 (B side, i); lanes 0..31 hold 32
     // consecutive i of one j, so packed stores are contiguous runs.
-    const bool interior = (I > J) && (i0 + GBM <= n);   // every pair valid, i > j
+    const bool interior = (I > J) && (i0 + BB <= n);   // every pair valid, i > j
     const int li0 = wc * 64 + (lane & 31);
     float ni[2], sb[2];
 #pragma unroll
@@ -312,7 +355,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
                 const unsigned dc = a * 32 + 8 * (r >> 2) + (r & 3);
                 const unsigned base_u = (LAYOUT == OUT_PACKED) ? dc * cw - dc * (dc + 1) / 2 : dc * cw;
                 const unsigned step_u = (LAYOUT == OUT_PACKED) ? 4 * cw - 4 * dc - 10 : 4 * cw;
-                const unsigned roff = base_u + (hi ? step_u : 0u) + (unsigned)li0;
+                const unsigned ro = base_u + (hi ? step_u : 0u) + (unsigned)li0;
                 const int lj = wr * 64 + (int)dc + (hi ? 4 : 0);
                 const float nj = nA[lj];
                 const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
@@ -325,7 +368,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
                     const bool ok = d2 >= fmaf(tau, ss, GRAM_SQRT_MIN);
                     fmask |= ok ? 0ull : (1ull << ((a * 2 + b) * 16 + r));
                     const float v = sqrt_rn(d2);
-                    outw[roff + (unsigned)(b * 32)] = v;
+                    outw[ro + (unsigned)(b * 32)] = v;
                     if (LAYOUT == OUT_FULL) args.out[(i0 + li0 + b * 32) * args.ldd + jw + lj - wr * 64] = v;
                 }
             }
@@ -366,41 +409,44 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
 
     // ---- exact FP32 recompute of the flagged pairs -------------------------
     int total;
-    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
+    const int myoff = block_exclusive_scan<C::NW>(__popcll(fmask), scan_tmp, total);
     if (total == 0) return;
     // the epilogue's stores of flagged pairs land before the exact values
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int64_t ldx = args.ldx;
-    // Staging unit e = tid + 1024 q (q < FUNITS): side e >> 11, row
-    // (e >> 3) & 255, features (e & 7) * 4..+3 of a 32-feature chunk.  The
-    // accumulators are dead here, so their registers hold a ring of FRING
-    // chunks in flight (the pass is bound by load latency, not bandwidth).
+    // Staging unit e = tid + THREADS q (q < FUNITS): tile row e >> 3 (A side
+    // rows first, then B side), features (e & 7) * 4..+3 of a 32-feature
+    // chunk.  The accumulators are dead here, so their registers hold a ring
+    // of FRING chunks in flight (the pass is bound by load latency, not
+    // bandwidth).
     constexpr int FRING = 2;
-    const float *fsrc[FUNITS];
-    int fcol[FUNITS];
+    constexpr int FUNITS = C::FUNITS, FPER = C::FPER;
+    // the unit's column is (tid & 7) * 4 for every q (THREADS is a multiple
+    // of 8); rows are kept as indices (fewer registers than pointers)
+    int frow[FUNITS];
 #pragma unroll
     for (int q = 0; q < FUNITS; ++q) {
-        const int e = tid + GTHREADS * q;
-        int64_t g = ((e >> 11) ? i0 : j0) + ((e >> 3) & (GBM - 1));
-        g = g < n ? g : n - 1;
-        fsrc[q] = args.x + g * ldx;
-        fcol[q] = (e & 7) * 4;
+        const int row = (tid + THREADS * q) >> 3;
+        int64_t g = row < BA ? j0 + row : i0 + (row - BA);
+        frow[q] = (int)(g < n ? g : n - 1);
     }
+    const float *const fx = args.x + (tid & 7) * 4;
     const int nf = (int)((k + FBK - 1) / FBK);
     float4 fr0[FUNITS], fr1[FUNITS];
 #define GT_FLOAD(buf, chunk)                                                               \
     do {                                                                                   \
         const int64_t fb_ = (int64_t)(chunk) * FBK;                                        \
+        const int64_t kk = fb_ + (tid & 7) * 4;                                            \
         _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
-            const int64_t kk = fb_ + fcol[q];                                              \
+            const float *src_ = fx + (int64_t)frow[q] * ldx + fb_;                         \
             if (VEC4) {                                                                    \
-                buf[q] = kk < k ? *reinterpret_cast<const float4 *>(fsrc[q] + kk)          \
+                buf[q] = kk < k ? *reinterpret_cast<const float4 *>(src_)                  \
                                 : make_float4(0.f, 0.f, 0.f, 0.f);                         \
             } else {                                                                       \
                 float v_[4];                                                               \
                 _Pragma("unroll") for (int c_ = 0; c_ < 4; ++c_) v_[c_] =                  \
-                    (kk + c_ < k) ? fsrc[q][kk + c_] : 0.0f;                               \
+                    (kk + c_ < k) ? src_[c_] : 0.0f;                                       \
                 buf[q] = make_float4(v_[0], v_[1], v_[2], v_[3]);                          \
             }                                                                              \
         }                                                                                  \
@@ -411,17 +457,16 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     do {                                                                                   \
         __syncthreads();                                                                   \
         _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
-            const int e = tid + GTHREADS * q;                                              \
-            *reinterpret_cast<float4 *>(fS + ((e >> 11) * GBM + ((e >> 3) & (GBM - 1))) *  \
-                                                 FLD + (e & 7) * 4) = buf[q];              \
+            const int e = tid + THREADS * q;                                               \
+            *reinterpret_cast<float4 *>(fS + (e >> 3) * FLD + (e & 7) * 4) = buf[q];       \
         }                                                                                  \
         __syncthreads();                                                                   \
         if ((chunk) + FRING < nf) GT_FLOAD(buf, (chunk) + FRING);                          \
         _Pragma("unroll") for (int c = 0; c < FPER; ++c) {                                 \
-            if (tid + GTHREADS * c < cnt) {                                                \
+            if (tid + THREADS * c < cnt) {                                                 \
                 const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD); \
                 const float4 *rb =                                                         \
-                    reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD); \
+                    reinterpret_cast<const float4 *>(fS + (BA + (ent[c] & 0xffffu)) * FLD); \
                 float part = 0.0f;                                                         \
                 _Pragma("unroll") for (int q = 0; q < FBK / 4; ++q) {                      \
                     const float4 u = ra[q], v = rb[q];                                     \
@@ -464,7 +509,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
-            const int e = tid + GTHREADS * c;
+            const int e = tid + THREADS * c;
             ent[c] = e < cnt ? list[e] : 0u;
         }
         for (int ch = 0; ch < nf; ch += FRING) {
@@ -473,7 +518,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         }
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
-            if (tid + GTHREADS * c < cnt) {
+            if (tid + THREADS * c < cnt) {
                 const int64_t j = j0 + (ent[c] >> 16), i = i0 + (ent[c] & 0xffffu);
                 const float v = sqrtf(sum[c]);
                 if (LAYOUT == OUT_PACKED) {
@@ -490,32 +535,107 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
 #undef GT_FLOAD
 }
 
-// ||x_i||^2 per row (and, for FP16, the power-of-two row scale and its
-// inverse): one wave per row.
+// Column centre c_j of the Gram form.  Euclidean distances do not change
+// when every point is moved by the same vector, but the Gram form's error
+// scales with ||x||^2 + ||y||^2: centring the columns makes that as small as
+// the data's spread allows (U[0,1) data: 4x; N(0,1) + 100: 10^4 x).  c_j is
+// the column mean rounded to a power-of-two grid no coarser than 1/64 of the
+// column's range, so that x - c_j is often exact (integer columns keep
+// integer centres); a constant column takes its value; a column holding a
+// NaN or an infinity is not moved.
+constexpr int CS_ROWS = 256;    // rows per partial-statistics block
+
+__global__ __launch_bounds__(256) void colstats_partial_kernel(const float *__restrict__ x,
+                                                               int64_t n, int64_t k, int64_t ldx,
+                                                               float *__restrict__ psum,
+                                                               float *__restrict__ pmin,
+                                                               float *__restrict__ pmax)
+{
+    const int64_t r0 = (int64_t)blockIdx.x * CS_ROWS;
+    const int64_t r1 = (r0 + CS_ROWS < n) ? r0 + CS_ROWS : n;
+    for (int64_t c = (int64_t)blockIdx.y * 256 + threadIdx.x; c < k; c += (int64_t)gridDim.y * 256) {
+        float sum = 0.0f, mn = __builtin_inff(), mx = -__builtin_inff();
+#pragma unroll 8
+        for (int64_t r = r0; r < r1; ++r) {
+            const float v = x[r * ldx + c];
+            sum += v;              // a NaN or an infinity makes the sum non-finite
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+        psum[blockIdx.x * k + c] = sum;
+        pmin[blockIdx.x * k + c] = mn;
+        pmax[blockIdx.x * k + c] = mx;
+    }
+}
+
+__global__ __launch_bounds__(256) void colstats_finish_kernel(const float *__restrict__ psum,
+                                                              const float *__restrict__ pmin,
+                                                              const float *__restrict__ pmax,
+                                                              int64_t blocks, int64_t n, int64_t k,
+                                                              float *__restrict__ center)
+{
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < k; c += (int64_t)gridDim.x * 256) {
+        float sum = 0.0f, mn = __builtin_inff(), mx = -__builtin_inff();
+        for (int64_t b = 0; b < blocks; ++b) {
+            sum += psum[b * k + c];
+            mn = fminf(mn, pmin[b * k + c]);
+            mx = fmaxf(mx, pmax[b * k + c]);
+        }
+        float cc = 0.0f;
+        const float range = mx - mn;
+        if (isfinite(sum) && isfinite(range)) {
+            if (range == 0.0f) {
+                cc = mn;
+            } else {
+                const float mean = sum / (float)n;
+                const float g = ldexpf(1.0f, ilogbf(range) - 6);   // <= range / 64
+                const float q = mean / g;
+                cc = (fabsf(q) < 0x1p24f) ? rintf(q) * g : mean;
+            }
+        }
+        center[c] = cc;
+    }
+}
+
+// ||x_i - c||^2 per row (and, for FP16, the power-of-two row scale and its
+// inverse): one wave per row.  The squares are summed with compensation
+// (each product split exactly by an fma, each addition's rounding error
+// recovered by TwoSum), so the norm is within about 2u of exact whatever k.
 __global__ __launch_bounds__(256) void row_prep_kernel(const float *__restrict__ x, int64_t n,
                                                        int64_t k, int64_t ldx,
+                                                       const float *__restrict__ center,
                                                        float *__restrict__ norms,
                                                        float *__restrict__ scales,
                                                        float *__restrict__ iscales)
 {
+#pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int64_t row = (int64_t)blockIdx.x * 4 + wave; row < n; row += (int64_t)gridDim.x * 4) {
         const float *xr = x + row * ldx;
-        float s = 0.0f, m = 0.0f;
+        float s = 0.0f, comp = 0.0f, m = 0.0f;
         for (int64_t c = lane; c < k; c += 64) {
-            const float v = xr[c];
-            s = fmaf(v, v, s);
+            const float v = xr[c] - center[c];
+            const float p = v * v;
+            const float pe = fmaf(v, v, -p);          // v * v = p + pe exactly
+            const float t = s + p;
+            const float bp = t - s;
+            comp += ((s - (t - bp)) + (p - bp)) + pe;  // s + p = t + (TwoSum error)
+            s = t;
             m = fmaxf(m, fabsf(v));
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
-            s += __shfl_xor(s, d, 64);
+            const float so = __shfl_xor(s, d, 64), co = __shfl_xor(comp, d, 64);
+            const float t = s + so;
+            const float bp = t - s;
+            comp = (comp + co) + ((s - (t - bp)) + (so - bp));
+            s = t;
             m = fmaxf(m, __shfl_xor(m, d, 64));
         }
         if (lane == 0) {
-            norms[row] = s;
+            norms[row] = s + comp;
             if (scales) {
-                // power of two that brings max|x| into [2^14, 2^15)
+                // power of two that brings max|x - c| into [2^14, 2^15)
                 int e = 0;
                 if (m > 0.0f) {
                     e = ilogbf(m);
@@ -529,6 +649,11 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const float *__restrict__
     }
 }
 
+// The 256 x 256 tile (WA = 4).  Two workgroups per CU on 128 x 256 tiles
+// (WA = 2) produce the same results bit for bit but measured slower on
+// 65536 x 512: BF16 19.0 vs 16.0 ms on the mixture, FP16 14.0 vs 11.9 ms.
+constexpr int kGramWA = 4;
+
 template <int SPLIT, int LAYOUT, bool VEC4>
 hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
 {
@@ -536,8 +661,8 @@ hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
         args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
-        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4>), dim3((unsigned)args.ids_this),
-                           dim3(GTHREADS), 0, stream, args);
+        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4, kGramWA>),
+                           dim3((unsigned)args.ids_this), dim3(kGramWA * 256), 0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -545,15 +670,15 @@ hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
 }
 
 template <int SPLIT>
-hipError_t launch_split(int layout, bool vec4, GramArgs &args, const float *scales, int64_t ids,
-                        hipStream_t stream)
+hipError_t launch_split(int layout, bool vec4, GramArgs &args, const float *center,
+                        const float *scales, int64_t ids, hipStream_t stream)
 {
     {
         const int64_t units = args.n * (args.kp / 8);
         int64_t blocks = (units + 255) / 256;
         if (blocks > 65536) blocks = 65536;
         hipLaunchKernelGGL(split_kernel<SPLIT>, dim3((unsigned)blocks), dim3(256), 0, stream, args.x,
-                           args.n, args.k, args.ldx, args.kp, scales,
+                           args.n, args.k, args.ldx, args.kp, center, scales,
                            const_cast<unsigned short *>(args.terms));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -573,25 +698,34 @@ int split_terms(int precision) { return precision == 1 ? 3 : 2; }
 
 }  // namespace
 
+int64_t colstats_blocks(int64_t n) { return (n + CS_ROWS - 1) / CS_ROWS; }
+
 // Workspace: split terms (NT x n x kp 16-bit values), then norms, scales and
-// inverse scales (n floats each).
+// inverse scales (n floats each), the column centres (k floats) and the
+// partial column statistics (3 x ceil(n / 256) x k floats).
 size_t gram_workspace_bytes(int64_t n, int64_t k, int precision)
 {
     const size_t terms = align256((size_t)split_terms(precision) * (size_t)n *
                                   (size_t)padded_k(k) * sizeof(unsigned short));
-    return terms + 3 * align256((size_t)n * sizeof(float));
+    return terms + 3 * align256((size_t)n * sizeof(float)) + align256((size_t)k * sizeof(float)) +
+           3 * align256((size_t)colstats_blocks(n) * (size_t)k * sizeof(float));
 }
 
-// Cancellation threshold: pairs with d^2 < tau (||x||^2 + ||y||^2) are
-// recomputed from exact differences.  tau bounds the Gram-form error of d^2,
-// in units of ||x||^2 + ||y||^2 (u = 2^-24):
+// Cancellation threshold: pairs with d^2 < tau (||x||^2 + ||y||^2) (norms
+// of the centred rows) are recomputed from exact differences of the original
+// rows.  The Gram-form error of d^2, in units of ||x||^2 + ||y||^2
+// (u = 2^-24):
 //   split    BF16 three terms: 4u; FP16 two terms: 12u (= 3 * 2^-22)
 //            (|dg| <= c sum|x_c y_c| <= c (||x||^2+||y||^2)/2 and |d(d^2)| = 2|dg|)
 //   accumulation of the MFMA partial sums, one FP32 rounding per product
 //            instruction: statistical bound 4 sqrt(#adds) u
-//   norms    (ceil(k/64) + 6) u for the two row norms together
+//   norms    4u for the two compensated row norms together
 //   final    3u for ||x||^2 + ||y||^2 - 2g
-// and divides it by 1.6e-5, the allowed relative error of d^2 (8e-6 on d).
+// Relative to d^2 of a pair that is not flagged (d^2 >= tau (...)), that is
+// at most E / tau.  Centring moves each difference x_c - y_c by at most
+// u (|x~_c| + |y~_c|), so d^2 by at most 2u d (||x~|| + ||y~||) <=
+// 2u sqrt(2 / tau) d^2 <= 8u d^2 for tau >= 1/8.  tau is chosen so that
+// E / tau + 8u = 1.6e-5, the allowed relative error of d^2 (8e-6 on d).
 float gram_tau(int precision, int64_t k)
 {
     const double u = 1.0 / 16777216.0;
@@ -599,15 +733,22 @@ float gram_tau(int precision, int64_t k)
     const double split = bf16 ? 4.0 * u : 12.0 * u;
     const double nadds = (bf16 ? 6.0 : 3.0) * (double)((k + 15) / 16);
     const double accum = 4.0 * sqrt(nadds) * u;
-    const double norms = ((double)((k + 63) / 64) + 6.0) * u;
-    const double total = split + accum + norms + 3.0 * u;
-    return (float)(total / 1.6e-5);
+    const double total = split + accum + 4.0 * u + 3.0 * u;
+    const double tau = total / (1.6e-5 - 8.0 * u);
+    return (float)(tau > 0.125 ? tau : 0.125);
 }
+
+// Above this tau nearly every pair would be recomputed: the request is
+// computed by the FP32 difference kernel instead.
+constexpr float GRAM_TAU_MAX = 0.9f;
 
 hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int64_t k,
                        int64_t ldx, float *out, int64_t ldd, void *workspace, hipStream_t stream)
 {
     if (n < 2 && layout == OUT_PACKED) return hipSuccess;
+    const float tau = gram_tau(precision, k);
+    if (tau > GRAM_TAU_MAX)
+        return launch_plain(OP_EUCLID, layout, x, n, ldx, x, n, ldx, k, 2.0f, out, ldd, stream);
     const int64_t kp = padded_k(k);
     char *ws = static_cast<char *>(workspace);
     unsigned short *terms = reinterpret_cast<unsigned short *>(ws);
@@ -617,12 +758,31 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
     float *scales = reinterpret_cast<float *>(ws);
     ws += align256((size_t)n * sizeof(float));
     float *iscales = reinterpret_cast<float *>(ws);
+    ws += align256((size_t)n * sizeof(float));
+    float *center = reinterpret_cast<float *>(ws);
+    ws += align256((size_t)k * sizeof(float));
+    const int64_t csb = colstats_blocks(n);
+    const size_t pbytes = align256((size_t)csb * (size_t)k * sizeof(float));
+    float *psum = reinterpret_cast<float *>(ws);
+    float *pmin = reinterpret_cast<float *>(ws + pbytes);
+    float *pmax = reinterpret_cast<float *>(ws + 2 * pbytes);
     const bool fp16 = (precision == 2);
+    {
+        const int64_t cb = (k + 255) / 256;
+        hipLaunchKernelGGL(colstats_partial_kernel, dim3((unsigned)csb, (unsigned)(cb < 1024 ? cb : 1024)),
+                           dim3(256), 0, stream, x, n, k, ldx, psum, pmin, pmax);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(colstats_finish_kernel, dim3((unsigned)(cb < 4096 ? cb : 4096)), dim3(256), 0,
+                           stream, psum, pmin, pmax, csb, n, k, center);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     {
         int64_t blocks = (n + 3) / 4;
         if (blocks > 65536) blocks = 65536;
         hipLaunchKernelGGL(row_prep_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, k,
-                           ldx, norms, fp16 ? scales : nullptr, fp16 ? iscales : nullptr);
+                           ldx, center, norms, fp16 ? scales : nullptr, fp16 ? iscales : nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -638,13 +798,13 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
     args.out = out;
     args.ldd = ldd;
     args.T = (n + GBM - 1) / GBM;
-    args.tau = gram_tau(precision, k);
+    args.tau = tau;
     const int64_t S = (args.T + GSUPER - 1) / GSUPER;
-    const int64_t ids = S * (S + 1) / 2 * GSUPER * GSUPER;
+    const int64_t ids = S * (S + 1) / 2 * GSUPER * GSUPER * (4 / kGramWA);
     const bool vec4 = (k % 4 == 0) && (ldx % 4 == 0) &&
                       ((reinterpret_cast<uintptr_t>(x) & 15u) == 0);
-    return fp16 ? launch_split<SPLIT_FP16>(layout, vec4, args, scales, ids, stream)
-                : launch_split<SPLIT_BF16>(layout, vec4, args, scales, ids, stream);
+    return fp16 ? launch_split<SPLIT_FP16>(layout, vec4, args, center, scales, ids, stream)
+                : launch_split<SPLIT_BF16>(layout, vec4, args, center, scales, ids, stream);
 }
 
 }  // namespace gt

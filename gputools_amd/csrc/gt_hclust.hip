@@ -597,7 +597,7 @@ __global__ __launch_bounds__(HTHREADS) void hclust_merge_kernel(HclustArgs a)
 
 hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
                          float *count, float *minv, int *minc, int *sub, int *sup, float *height,
-                         hipStream_t stream)
+                         int state, hipStream_t stream)
 {
     if (n < 2) return hipSuccess;
     HclustArgs a;
@@ -621,7 +621,9 @@ hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lam
     if (e != hipSuccess) return e;
     // 16 B of LDS per row (count, cached minimum and its column, the new row
     // sup) next to the 8 KB of static LDS: 155 KB at the largest n
-    const bool lds = n <= HC_LDS_ROWS;
+    static_assert(HC_LDS_ROWS == HC_LDS_MAX_N, "LDS row limit");
+    if (state == HC_STATE_LDS && n > HC_LDS_ROWS) return hipErrorInvalidValue;
+    const bool lds = state == HC_STATE_LDS || (state == HC_STATE_AUTO && n <= HC_LDS_ROWS);
     const unsigned shm = lds ? (unsigned)(16 * n) : 0u;
     switch (method) {
 #define GT_HC_CASE(m)                                                                          \

@@ -89,9 +89,14 @@ enum HcMethod {
 
 // Clustering on a full n x n device matrix d (row stride ldd, destroyed).
 // count, minv, minc: n entries of workspace each; sub, sup, height: n - 1.
+// state: HC_STATE_AUTO keeps the per-row state in LDS up to HC_LDS_MAX_N
+// points and in global memory above; HC_STATE_LDS / HC_STATE_GLOBAL force a
+// placement (LDS only up to HC_LDS_MAX_N).
+enum HcState { HC_STATE_AUTO = 0, HC_STATE_LDS = 1, HC_STATE_GLOBAL = 2 };
+constexpr int64_t HC_LDS_MAX_N = 9216;
 hipError_t launch_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
                          float *count, float *minv, int *minc, int *sub, int *sup, float *height,
-                         hipStream_t stream);
+                         int state, hipStream_t stream);
 
 // Pearson correlation (gpuCor).  use: 0 everything, 1 pairwise.complete.obs.
 // precision (use 0 only): 0 FP32 VALU, 1 BF16 three-term, 2 FP16 two-term

@@ -145,11 +145,18 @@ def hclust_method_id(method) -> int:
     return int(method)
 
 
-def hclust(d, method="complete", lam: float = 0.5, beta: float = 0.5, stream=None):
+HC_STATES = {"auto": 0, "lds": 1, "global": 2}
+
+
+def hclust(d, method="complete", lam: float = 0.5, beta: float = 0.5, stream=None,
+           state: str = "auto"):
     """Clustering stage on a full n x n FP32 device matrix ``d`` (torch
     tensor on the GPU, consumed: it is overwritten).  Returns device int32
     tensors ``sub``, ``sup`` and float32 ``height`` of length n - 1: at step s
-    the clusters represented by sub[s] < sup[s] merge at height[s]."""
+    the clusters represented by sub[s] < sup[s] merge at height[s].
+    ``state`` places the kernel's per-row state: "auto" (LDS up to 9216
+    points, global memory above), "lds" or "global"; every placement gives
+    the same result."""
     import torch
     if not isinstance(d, torch.Tensor) or not d.is_cuda or d.dtype != torch.float32:
         raise TypeError("d must be a float32 torch tensor on the GPU")
@@ -164,10 +171,11 @@ def hclust(d, method="complete", lam: float = 0.5, beta: float = 0.5, stream=Non
     check(lib().gt_hclust_workspace_size(n, ctypes.byref(b)))
     ws = torch.empty(max(b.value, 1), dtype=torch.uint8, device=d.device)
     s = stream if stream is not None else torch.cuda.current_stream()
-    check(lib().gt_hclust(d.data_ptr(), n, d.stride(0), hclust_method_id(method), float(lam),
-                          float(beta), sub.data_ptr() if m else None,
-                          sup.data_ptr() if m else None, height.data_ptr() if m else None,
-                          ws.data_ptr(), b.value, s.cuda_stream or None))
+    check(lib().gt_hclust_state(d.data_ptr(), n, d.stride(0), hclust_method_id(method),
+                                float(lam), float(beta), sub.data_ptr() if m else None,
+                                sup.data_ptr() if m else None,
+                                height.data_ptr() if m else None, ws.data_ptr(), b.value,
+                                HC_STATES[state], s.cuda_stream or None))
     return sub, sup, height
 
 

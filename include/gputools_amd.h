@@ -203,6 +203,7 @@ int gt_dist_ws(const float *x, int64_t n, int64_t k, int64_t ldx, int method, fl
                int precision, int layout, float *out, int64_t ldd,
                void *workspace, size_t workspace_bytes, void *stream);
 
+
 /* Cross distances between the rows of A (na x k) and B (nb x k), CROSS
  * layout.  The semantics of the unreachable *_kernel cross kernels
  * (inst/cuda/distance.cu:3-38, 80-114, 156-190, 234-271, 318-364, 417-451),
@@ -243,6 +244,26 @@ int gt_hclust_workspace_size(int64_t n, size_t *bytes);
  * and arithmetic follow the reference bit for bit.  n below 2^24. */
 int gt_hclust(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta, int *sub,
               int *sup, float *height, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Where the clustering kernel keeps its per-row state (counts, cached row
+ * minima, the rewritten row): GT_HC_STATE_AUTO as gt_hclust does (LDS up to
+ * GT_HC_LDS_MAX_N points, global memory above), or one placement forced (LDS
+ * only up to GT_HC_LDS_MAX_N).  Both placements run the same merge rounds and
+ * give the same merges and heights bit for bit. */
+enum gt_hc_state {
+    GT_HC_STATE_AUTO = 0,
+    GT_HC_STATE_LDS = 1,
+    GT_HC_STATE_GLOBAL = 2
+};
+#define GT_HC_LDS_MAX_N 9216
+
+/* gt_hclust with the per-row state placement chosen by the caller, so that
+ * either kernel variant can be run (and compared) at any size.  The
+ * reference has one placement (its kernels re-read the whole matrix every
+ * round, src/hcluster.cpp:40-201); this entry point has no counterpart. */
+int gt_hclust_state(float *d, int64_t n, int64_t ldd, int method, float lambda, float beta,
+                    int *sub, int *sup, float *height, void *workspace, size_t workspace_bytes,
+                    int state, void *stream);
 
 /* gt_hclust on a host matrix (the replacement of hcluster,
  * src/hcluster.cpp:15-38): host in, host sub/sup/height out, synchronous. */

@@ -62,8 +62,9 @@ def test_gram_small_integers_bit_exact(precision):
 
 @pytest.mark.parametrize("precision", SPLITS)
 def test_gram_all_pairs_near_coincident(precision):
-    """Every pair cancels (points within 1e-3 of a centre at 1e3): each tile
-    flags all of its pairs, which exercises several fix-up passes."""
+    """Points within 1e-3 of one centre at 1e3: without centring every pair
+    would cancel; the column centres bring the rows to their spread, so the
+    Gram form itself must be accurate here."""
     rng = np.random.default_rng(32)
     c = rng.standard_normal(96).astype(np.float32) * 1000
     x = c + rng.standard_normal((300, 96)).astype(np.float32) * np.float32(1e-3)
@@ -72,6 +73,66 @@ def test_gram_all_pairs_near_coincident(precision):
     want = orc.dist(x, "euclidean")
     assert got[orc.packed_index(300, 7, 3)] == 0.0
     assert max_rel(got, want) <= RTOL
+
+
+@pytest.mark.parametrize("precision", SPLITS)
+def test_gram_two_far_clusters_many_fixup_passes(precision):
+    """Two tight clusters 2e3 apart: centring cannot help the pairs inside a
+    cluster (they cancel against the clusters' offset), so half of every
+    tile's pairs are recomputed exactly, over many list passes."""
+    rng = np.random.default_rng(36)
+    c = rng.standard_normal(96).astype(np.float32) * 1000
+    side = np.where(np.arange(600) % 2 == 0, 1.0, -1.0).astype(np.float32)[:, None]
+    x = side * c + rng.standard_normal((600, 96)).astype(np.float32) * np.float32(1e-3)
+    x[9] = x[5]                   # an exact duplicate: distance 0
+    got = run(x, precision)
+    want = orc.dist(x, "euclidean")
+    assert got[orc.packed_index(600, 9, 5)] == 0.0
+    assert max_rel(got, want) <= RTOL
+
+
+@pytest.mark.parametrize("precision", SPLITS)
+@pytest.mark.parametrize("name,n,k,shift", [("uniform", 4096, 512, 0.0),
+                                            ("normal", 2000, 256, 100.0),
+                                            ("normal", 1500, 300, 1e4),
+                                            ("uniform", 3000, 64, 0.0)])
+def test_gram_non_zero_mean(precision, name, n, k, shift):
+    """Data far from the origin: the Gram form is taken about the column
+    centres, so accuracy (and the share of exactly recomputed pairs) does
+    not depend on the offset."""
+    x = getattr(wl, name)(n, k, seed=37) + np.float32(shift)
+    print("case gram offset", precision, name, n, k, shift, flush=True)
+    got = run(x, precision)
+    assert max_rel(got, orc.dist(x, "euclidean")) <= RTOL
+
+
+@pytest.mark.parametrize("precision", SPLITS)
+def test_gram_per_column_offsets(precision):
+    rng = np.random.default_rng(38)
+    x = (rng.standard_normal((1200, 200)) + rng.standard_normal(200) * 1e3).astype(np.float32)
+    x[:, 17] = 5.0                # a constant column
+    got = run(x, precision)
+    assert max_rel(got, orc.dist(x, "euclidean")) <= RTOL
+
+
+@pytest.mark.parametrize("precision", SPLITS)
+@pytest.mark.parametrize("n,k", [(300, 4096), (257, 8192), (130, 16384), (64, 65536),
+                                 (200, 3001)])
+def test_gram_large_k(precision, n, k):
+    """Many features: the accumulation term of the threshold grows like
+    sqrt(k); past the point where nearly every pair would be recomputed the
+    request runs on the FP32 difference kernel.  Every pair is checked."""
+    x = wl.normal(n, k, seed=39) + np.float32(0.5)
+    print("case gram large k", precision, n, k, flush=True)
+    got = run(x, precision)
+    assert max_rel(got, orc.dist(x, "euclidean")) <= RTOL
+
+
+@pytest.mark.parametrize("precision", SPLITS)
+def test_gram_large_k_mixture(precision):
+    x = wl.mixture(512, 8192, seed=40)
+    got = run(x, precision)
+    assert max_rel(got, orc.dist(x, "euclidean")) <= RTOL
 
 
 @pytest.mark.parametrize("precision", SPLITS)

@@ -26,15 +26,15 @@ def full_matrix(x):
     return gt.dist(xt, "euclidean", layout="full").cpu().numpy()
 
 
-def run_gpu(d, method):
+def run_gpu(d, method, state="auto", lam=0.5, beta=0.5):
     dt = torch.from_numpy(np.ascontiguousarray(d, dtype=np.float32)).cuda()
-    sub, sup, h = gt.hclust(dt, method)
+    sub, sup, h = gt.hclust(dt, method, lam, beta, state=state)
     torch.cuda.synchronize()
     return sub.cpu().numpy(), sup.cpu().numpy(), h.cpu().numpy()
 
 
-def assert_same(d, method):
-    sub_g, sup_g, h_g = run_gpu(d, method)
+def assert_same(d, method, state="auto"):
+    sub_g, sup_g, h_g = run_gpu(d, method, state)
     sub_o, sup_o, h_o = orc.hclust(d, method)
     np.testing.assert_array_equal(sub_g, sub_o)
     np.testing.assert_array_equal(sup_g, sup_o)
@@ -66,6 +66,57 @@ def test_hclust_matches_oracle_n2048(method):
     d = full_matrix(rng.standard_normal((2048, 16)))
     print("case hclust n=2048 %s" % method, flush=True)
     assert_same(d, method)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("n", [3, 65, 300, 2048])
+def test_hclust_global_state_matches_oracle(method, n):
+    """The variant that keeps the per-row state in global memory (the one
+    every n above 9216 runs) against the reference-order oracle, at sizes the
+    oracle reaches."""
+    rng = np.random.default_rng(2000 + n)
+    d = full_matrix(rng.standard_normal((n, 11)))
+    print("case hclust global state n=%d %s" % (n, method), flush=True)
+    assert_same(d, method, state="global")
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_hclust_global_state_with_ties(method):
+    rng = np.random.default_rng(78)
+    d = full_matrix(rng.integers(0, 3, size=(300, 3)))
+    print("case hclust global state ties n=300 %s" % method, flush=True)
+    assert_same(d, method, state="global")
+
+
+def test_hclust_global_state_clustered_data():
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal((8, 4)).astype(np.float32) * 10
+    x = c[np.arange(1500) % 8] + rng.standard_normal((1500, 4)).astype(np.float32)
+    d = full_matrix(x)
+    for method in ("single", "complete", "average", "ward", "centroid", "median"):
+        print("case hclust global state mixture n=1500 %s" % method, flush=True)
+        assert_same(d, method, state="global")
+
+
+@pytest.mark.parametrize("state", ["lds", "global"])
+@pytest.mark.parametrize("method,lam,beta", [("flexible", 0.25, -0.25),
+                                             ("flexible_group", 0.25, -0.25),
+                                             ("flexible", 0.7, 0.1)])
+def test_hclust_flexible_parameters(method, lam, beta, state):
+    """lambda and beta other than R's 0.5 / 0.5 reach the kernel unchanged."""
+    rng = np.random.default_rng(31)
+    d = full_matrix(rng.standard_normal((400, 5)))
+    sub_g, sup_g, h_g = run_gpu(d, method, state, lam, beta)
+    sub_o, sup_o, h_o = orc.hclust(d, method, lam, beta)
+    np.testing.assert_array_equal(sub_g, sub_o)
+    np.testing.assert_array_equal(sup_g, sup_o)
+    np.testing.assert_array_equal(h_g.view(np.uint32), h_o.view(np.uint32))
+
+
+def test_hclust_state_argument_checks():
+    d = torch.zeros((9300, 9300), dtype=torch.float32, device="cuda")
+    with pytest.raises(gt.GputoolsError, match="LDS placement"):
+        gt.hclust(d, "single", state="lds")
 
 
 def test_hclust_clustered_data_many_rescans():

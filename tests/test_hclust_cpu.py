@@ -69,6 +69,48 @@ def test_oracle_three_points_closed_form(method, expect):
     assert abs(float(h[1]) - expect(d01, d02, d12)) <= 1e-6 * abs(expect(d01, d02, d12))
 
 
+def _four_points():
+    # d01 = 1 merges first (sub 0, sup 1), then {0, 1} (now at index 1, size
+    # 2) with point 2, then that cluster of 3 with point 3: the second update
+    # combines clusters of unequal size
+    d = np.array([[0, 1, 4, 10],
+                  [1, 0, 5, 12],
+                  [4, 5, 0, 20],
+                  [10, 12, 20, 0]], np.float32)
+    return d
+
+
+# Heights worked by hand from the update kernels' text (inst/cuda/hcluster.cu,
+# ward at :380-388): when the cluster at index sub (np points) is absorbed
+# into sup (nq points), nr = np + nq, for a cluster k of nk points with
+# top = d(k, sub), bot = d(k, sup), val = the merge height:
+#   ward            (bot (np + nk) + top (nq + nk) - val nk) / (nr + nk)
+#   flexible        (bot (1 - lam) + top (1 - lam)) / 2 + beta val
+#   flexible group  (bot (1 - lam) np + top (1 - lam) nq) / nr + beta val
+# i.e. the surviving row is weighted by the absorbed cluster's size.  On the
+# four points (round 2 absorbs the size-2 cluster at index 1 into point 2):
+#   ward:  round 1 d(2,{01}) = (5*2 + 4*2 - 1)/3 = 17/3, d(3,{01}) = 43/3;
+#          round 3 height (20*3 + 43/3*2 - 17/3)/4 = 83/4 = 20.75
+#          (textbook weights would give (43/3*3 + 20*2 - 17/3)/4 = 19.33)
+#   lam = 0.25, beta = -0.25:
+#   flexible:       3.125, 8.0; then (20*.75 + 8*.75)/2 - 3.125/4 = 9.71875
+#   flexible group: 3.125, 8.0; then (20*.75*2 + 8*.75*1)/3 - 0.78125 = 11.21875
+#          (textbook weights would give 8.21875)
+@pytest.mark.parametrize("method,lam,beta,heights", [
+    ("ward", 0.5, 0.5, (1.0, 17.0 / 3.0, 20.75)),
+    ("flexible", 0.25, -0.25, (1.0, 3.125, 9.71875)),
+    ("flexible_group", 0.25, -0.25, (1.0, 3.125, 11.21875)),
+    # R's lambda = beta = 0.5: round 1 (5+4)/4 + 0.5 = 2.75, (12+10)/4 + 0.5 = 6;
+    # then flexible (20+6)/4 + 1.375 = 7.875, group (20*2 + 6)/6 + 1.375
+    ("flexible", 0.5, 0.5, (1.0, 2.75, 7.875)),
+    ("flexible_group", 0.5, 0.5, (1.0, 2.75, 46.0 / 6.0 + 1.375)),
+])
+def test_oracle_unequal_cluster_sizes_hand_worked(method, lam, beta, heights):
+    sub, sup, h = orc.hclust(_four_points(), method, lam, beta)
+    assert list(sub) == [0, 1, 2] and list(sup) == [1, 2, 3]
+    np.testing.assert_allclose(h.astype(np.float64), heights, rtol=1e-6, atol=0)
+
+
 def _random_sequence(rng, n):
     active, sub, sup = list(range(n)), [], []
     for _ in range(n - 1):
