@@ -18,6 +18,8 @@ METHOD_NAMES = ("euclidean", "maximum", "manhattan", "canberra", "binary", "mink
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp16x3": PREC_FP16X3}
 LAYOUT_PACKED, LAYOUT_FULL = 0, 1
+# GT_NOALLOC_SCRATCH_BYTES: the static scratch of gt_dist_noalloc, per GPU
+NOALLOC_SCRATCH_BYTES = 64 << 20
 # gt_hc_method (hc_method order, src/hcluster.h:5-16); names as getClusterEnum spells them
 HC_METHOD_NAMES = ("single", "complete", "wpgma", "average", "median", "centroid",
                    "flexible_group", "flexible", "ward", "mcquitty")
@@ -65,6 +67,8 @@ SIGNATURES = {
     "gt_dist": (c_int, [c_p, c_i64, c_i64, c_i64, c_int, c_float, c_int, c_int, c_p, c_i64, c_p]),
     "gt_dist_ws": (c_int, [c_p, c_i64, c_i64, c_i64, c_int, c_float, c_int, c_int, c_p, c_i64,
                            c_p, c_size, c_p]),
+    "gt_dist_noalloc": (c_int, [c_p, c_i64, c_i64, c_i64, c_int, c_float, c_int, c_int, c_p, c_i64,
+                                c_p]),
     "gt_dist_cross": (c_int, [c_p, c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_int, c_float, c_p,
                               c_i64, c_p]),
     "gt_dist_host": (c_int, [c_p, c_i64, c_i64, c_int, c_float, c_int, c_int, c_p]),

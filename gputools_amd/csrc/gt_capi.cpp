@@ -19,7 +19,14 @@
 
 #include <algorithm>
 #include <limits>
+#include <mutex>
 #include <vector>
+
+// Workspace of gt_dist_noalloc: a static device array, one copy per GPU,
+// held from module load until the process ends.  Its name is a host-side
+// stand-in; host code uses only the device address that hipGetSymbolAddress
+// returns for the current device (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[GT_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -172,7 +179,8 @@ int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, floa
 // synchronous hipMalloc/hipFree pair only where the stream-ordered pool is
 // not supported, and never while the stream is being captured into a graph
 // (a synchronous allocation or synchronisation would invalidate the capture):
-// callers that capture pass their own workspace (the *_ws entry points).
+// callers that capture pass their own workspace (the *_ws entry points) or
+// use gt_dist_noalloc.
 struct Scratch {
     void *ptr = nullptr;
     bool async = false;
@@ -193,7 +201,7 @@ struct Scratch {
         if (hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone)
             return fail(GT_ERR_WORKSPACE,
                         "stream-ordered workspace allocation failed during graph capture (%s); "
-                        "pass a workspace (gt_dist_ws)",
+                        "pass a workspace (gt_dist_ws) or call gt_dist_noalloc",
                         hipGetErrorString(e));
         if (e != hipErrorNotSupported) return hip_fail(e, "hipMallocAsync(workspace)");
         e = hipMalloc(&ptr, bytes);
@@ -219,6 +227,28 @@ struct Scratch {
         return e == hipSuccess ? GT_OK : hip_fail(e, "releasing workspace");
     }
 };
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out)
+{
+    int dev = 0;
+    GT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GT_ERR_INVALID, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        GT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) return fail(GT_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GT_OK;
+}
 
 struct DeviceBuffer {
     void *ptr = nullptr;
@@ -582,6 +612,26 @@ int gt_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float
     return s != GT_OK ? s : r;
 }
 
+int gt_dist_noalloc(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+                    int precision, int layout, float *out, int64_t ldd, void *stream)
+{
+    int s = check_dist_args(x, n, k, ldx, method, p, precision, layout, out, ldd);
+    if (s != GT_OK) return s;
+    size_t need = 0;
+    workspace_bytes(n, k, method, precision, &need);
+    if (need > GT_NOALLOC_SCRATCH_BYTES)
+        return fail(GT_ERR_WORKSPACE,
+                    "gt_dist_noalloc: this call needs %zu bytes of workspace, above the %zu of "
+                    "its static scratch; call gt_dist_ws with a workspace instead",
+                    need, (size_t)GT_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    if (need > 0 && (s = scratch_address(&ws)) != GT_OK) return s;
+    // every kernel on this path writes its workspace in full before reading
+    // it (split terms and padding, norms, bit rows and counts), so the
+    // previous call's contents need no clearing
+    return run_dist(x, n, k, ldx, method, p, precision, layout, out, ldd, ws,
+                    static_cast<hipStream_t>(stream));
+}
 
 int gt_dist_cross(const float *a, int64_t na, int64_t lda, const float *b, int64_t nb,
                   int64_t ldb, int64_t k, int method, float p, float *out, int64_t ldd,

@@ -203,6 +203,23 @@ int gt_dist_ws(const float *x, int64_t n, int64_t k, int64_t ldx, int method, fl
                int precision, int layout, float *out, int64_t ldd,
                void *workspace, size_t workspace_bytes, void *stream);
 
+/* Bytes of the static device scratch behind gt_dist_noalloc (per GPU). */
+#define GT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* gt_dist without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The workspace
+ * gt_dist_workspace_size reports comes from a static device array of
+ * GT_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the
+ * process ends; a request that needs more returns GT_ERR_WORKSPACE (use
+ * gt_dist_ws).  The plain FP32 methods other than binary need no workspace;
+ * binary needs n * (ceil(k/32) + 1) * 4 bytes, the split-precision forms
+ * about n * k * 6 (BF16X3) or n * k * 4 (FP16X3) bytes.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two host threads) race and must be serialised by the caller.  Results are
+ * those of gt_dist, bit for bit. */
+int gt_dist_noalloc(const float *x, int64_t n, int64_t k, int64_t ldx, int method, float p,
+                    int precision, int layout, float *out, int64_t ldd, void *stream);
 
 /* Cross distances between the rows of A (na x k) and B (nb x k), CROSS
  * layout.  The semantics of the unreachable *_kernel cross kernels

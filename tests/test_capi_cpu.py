@@ -163,3 +163,24 @@ def test_dist_object_layout():
     assert d.attrs["Diag"] is False and d.attrs["Upper"] is False and "p" not in d.attrs
     with pytest.raises(IndexError):
         d.index(3, 3)
+
+
+def test_noalloc_rejects_oversized_requests_before_the_device():
+    """gt_dist_noalloc: a request whose workspace exceeds the static scratch
+    fails with GT_ERR_WORKSPACE, naming the limit, without any device call."""
+    L = _lib.lib()
+    dummy = ctypes.c_void_p(16)
+    n, k = 65536, 512
+    need = gt.workspace_size(n, k, "euclidean", "bf16x3")
+    assert need > _lib.NOALLOC_SCRATCH_BYTES
+    s, msg = _status(L.gt_dist_noalloc, dummy, n, k, k, 0, 2.0, _lib.PREC_BF16X3,
+                     _lib.LAYOUT_PACKED, dummy, n, None)
+    assert s == _lib.GT_ERR_WORKSPACE, msg
+    assert str(_lib.NOALLOC_SCRATCH_BYTES) in msg and "gt_dist_ws" in msg
+    # binary rows of 2^20 points x 4096 features: 129 words each, 541 MB
+    s, msg = _status(L.gt_dist_noalloc, dummy, 1 << 20, 4096, 4096, _lib.BINARY, 2.0,
+                     _lib.PREC_FP32, _lib.LAYOUT_PACKED, dummy, 1 << 20, None)
+    assert s == _lib.GT_ERR_WORKSPACE, msg
+    # ordinary argument errors come first, as for gt_dist
+    s, msg = _status(L.gt_dist_noalloc, dummy, 10, 4, 4, 9, 2.0, 0, 0, dummy, 10, None)
+    assert s == _lib.GT_ERR_INVALID and "method" in msg
