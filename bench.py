@@ -52,15 +52,19 @@ def parse():
                     help="skip the secondary paths (other methods, batched, backward, MDS, ...)")
     ap.add_argument("--paths", default=None,
                     help="comma list of secondary path groups to time (default: all of "
-                         "methods,batched,backward,small,hclust,pmcc,kendall)")
+                         "methods,binary_large,uniform,batched,backward,small,hclust,pmcc,"
+                         "kendall)")
     ap.add_argument("--paths-only", action="store_true",
                     help="time only the selected secondary paths (no headline line; for "
                          "profiling them)")
-    ap.add_argument("--host-path", action="store_true",
-                    help="also time the host-buffer entry point (PCIe-inclusive; not `value`)")
+    ap.add_argument("--no-host-path", action="store_true",
+                    help="skip timing the host-buffer entry point (PCIe-inclusive; never `value`)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (reference kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--exhaustive", action="store_true",
+                    help="also compare every one of the n(n-1)/2 distances of each timed "
+                         "precision with the FP64 oracle (streamed on the host cores; minutes)")
     ap.add_argument("--precisions", default=None,
                     help="comma list of paths to time (default: the headline and fp32)")
     return ap.parse_args()
@@ -103,6 +107,63 @@ def sampled_pairs(n, rows):
     gi, gj = r[ii], r[jj]
     idx = gj * n - gj * (gj + 1) // 2 + (gi - gj - 1)
     return r, idx
+
+
+def host_facts():
+    """The CPU the baseline runs on: logical CPUs, the ones this process may
+    use, the thread share (OMP_NUM_THREADS when set), the lscpu model, and
+    whether R is installed (R's stats::dist is the named CPU baseline; it is
+    timed only where Rscript exists, and nothing is installed)."""
+    import shutil
+    import subprocess
+    facts = {"nproc": os.cpu_count()}
+    try:
+        facts["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        facts["affinity"] = facts["nproc"]
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    facts["threads_available"] = int(omp) if omp.isdigit() and int(omp) > 0 else facts["affinity"]
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                facts["model"] = ln.split(":", 1)[1].strip()
+            elif ln.startswith("Socket(s):"):
+                facts["sockets"] = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    rs = shutil.which("Rscript")
+    if rs is None:
+        facts["Rscript"] = "absent (Rscript --version: command not found)"
+    else:
+        try:
+            r = subprocess.run([rs, "--version"], capture_output=True, text=True, timeout=30)
+            facts["Rscript"] = (r.stdout + r.stderr).strip()[:200]
+        except (OSError, subprocess.SubprocessError) as e:
+            facts["Rscript"] = "present but failed: %s" % e
+    return facts
+
+
+def time_r_stats_dist(x):
+    """Seconds R's stats::dist takes on x (rows = points), or None when R is
+    not installed.  The points go to R through a binary file (readBin), so
+    R's own RNG is never involved."""
+    import shutil
+    import subprocess
+    import tempfile
+    rs = shutil.which("Rscript")
+    if rs is None:
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "x.bin")
+        x.astype(np.float64).tofile(f)
+        expr = ("x <- matrix(readBin('%s', 'double', n = %d), nrow = %d, byrow = TRUE); "
+                "t <- system.time(d <- dist(x)); cat(t[['elapsed']])" % (f, x.size, x.shape[0]))
+        r = subprocess.run([rs, "-e", expr], capture_output=True, text=True, timeout=600)
+        try:
+            return float(r.stdout.strip().split()[-1])
+        except (ValueError, IndexError):
+            return None
 
 
 def _timed(fn, steps, warmup, torch, stream=None):
@@ -159,6 +220,45 @@ def path_methods(torch, gt, wl, steps, warmup, out):
     out["methods_8192x256_fp32"] = meth
 
 
+def path_binary_large(torch, gt, wl, steps, warmup, out):
+    # binary at 65536 x 256 (Bernoulli(0.1)): 8.59 GB of packed output, so
+    # the HBM write of the result bounds it (1.07 ms at 8 TB/s)
+    n, k = 65536, 256
+    pairs = n * (n - 1) // 2
+    x = torch.from_numpy(wl.bernoulli(n, k, seed=2)).cuda()
+    o = torch.empty(pairs, dtype=torch.float32, device="cuda")
+    ms = _timed(lambda: gt.dist(x, "binary", out=o), steps, warmup, torch)
+    gbs = (4.0 * pairs + 4.0 * n * k) / (ms * 1e-3) / 1e9
+    out["binary_65536x256"] = {
+        "ms": round(ms, 4), "pairs_per_s": round(pairs / (ms * 1e-3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4)}}
+
+
+def path_uniform(torch, gt, wl, steps, warmup, out):
+    # euclidean on U[0,1) data, 65536 x 512: non-zero column means, which the
+    # Gram form takes out (column centres) before splitting.  Error against
+    # the FP64 oracle on every pair among 2000 seeded rows.
+    from oracle import oracle as orc
+    n, k = 65536, 512
+    pairs = n * (n - 1) // 2
+    xh = wl.uniform(n, k, seed=1)
+    x = torch.from_numpy(xh).cuda()
+    o = torch.empty(pairs, dtype=torch.float32, device="cuda")
+    rows, idx = sampled_pairs(n, np.random.default_rng(7).choice(n, 2000, replace=False))
+    ref = orc.dist(np.ascontiguousarray(xh[rows]), "euclidean")
+    rec = {}
+    for prec in ("bf16x3", "fp16x3", "fp32"):
+        ms = _timed(lambda: gt.dist(x, "euclidean", precision=prec, out=o), steps, warmup, torch)
+        got = o[torch.from_numpy(idx).cuda()].cpu().numpy().astype(np.float64)
+        rec[prec] = {"ms": round(ms, 4), "pairs_per_s": round(pairs / (ms * 1e-3), 1),
+                     "max_rel_err": float("%.3g" % np.max(np.abs(got - ref) / ref))}
+    for prec in ("bf16x3", "fp16x3"):
+        rec[prec]["speedup_vs_fp32"] = round(rec["fp32"]["ms"] / rec[prec]["ms"], 3)
+    rec["error_sample_pairs"] = int(ref.size)
+    out["uniform_65536x512"] = rec
+
+
 def path_batched(torch, gt, wl, steps, warmup, out):
     # batched: 4096 sets, log-uniform sizes in [32, 2048], d in {16, 32, 64, 128}
     from gputools_amd.autotune import tune_batched
@@ -192,9 +292,12 @@ def path_backward(torch, gt, wl, steps, warmup, out):
     delta = gt.dist(x)
     grad = torch.ones_like(delta)
     ms_bw = _timed(lambda: gt.dist_backward(x, grad, "euclidean"), steps, warmup, torch)
-    # per pair and owner: 3k (recompute) + 2k (accumulate); each pair has two owners
+    # algorithmic work per pair: about 6k flop (the difference, its square and
+    # sum to recompute d, and the weighted difference added to both rows' dX);
+    # D is recomputed, not read.  The kernel's two-owner sweep does more.
     out["backward_euclidean_16384x128"] = {
-        "ms": round(ms_bw, 4), "roofline": _valu_roof(2.0 * pairs * 5 * k, ms_bw)}
+        "ms": round(ms_bw, 4), "algorithmic_flops": 6.0 * k * pairs,
+        "roofline": _valu_roof(6.0 * pairs * k, ms_bw)}
     mds = {}
     for q in (2, 32):
         y = torch.from_numpy(wl.normal(n, q, seed=44)).cuda()
@@ -203,9 +306,11 @@ def path_backward(torch, gt, wl, steps, warmup, out):
             yy = y.detach().requires_grad_(True)
             gt.mds_stress(yy, delta).backward()
         ms = _timed(step, steps, warmup, torch)
-        # bytes: delta read by both owners of every pair (4 B each time)
+        # algorithmic work per pair: about 6q flop (as the backward) and one
+        # 4-byte read of delta
         mds["q=%d" % q] = {"ms_forward_backward": round(ms, 4),
-                           "roofline": _roof(2.0 * pairs * 5 * q, 2.0 * pairs * 4, ms)}
+                           "algorithmic_flops": 6.0 * q * pairs, "algorithmic_bytes": 4.0 * pairs,
+                           "roofline": _roof(6.0 * pairs * q, 4.0 * pairs, ms)}
     out["mds_stress_16384"] = mds
 
 
@@ -223,20 +328,36 @@ def path_hclust(torch, gt, wl, steps, warmup, out):
     # figure of merit is time per round.  The reference's loop rescans the
     # lower triangle every round: n^3/6 FP32 reads, whose HBM time alone is
     # given as the reference's floor.
-    n = 8192
-    x = torch.from_numpy(wl.normal(n, 256, seed=2)).cuda()
-    d0 = gt.dist(x, "euclidean", layout="full")
-    hc = {"reference_floor_ms": round(n ** 3 / 6 * 4 / 8e12 * 1e3, 2)}
-    for m in ("single", "complete", "average", "ward"):
-        d = d0.clone()
+    # Our kernel reads O(n^2) in all, so it is bound by the latency of its
+    # n - 1 dependent rounds (argmin, update, repair, rescan, each ending in a
+    # barrier), not by bandwidth.  The floor per round is measured with the
+    # same kernel at n = 64, where every phase has almost nothing to do.
+    def run_rounds(d, m):
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         gt.hclust(d, m)
         e1.record()
         torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1)
-        hc[m] = {"ms": round(ms, 2), "us_per_round": round(1e3 * ms / (n - 1), 2)}
+        return e0.elapsed_time(e1)
+
+    small = gt.dist(torch.from_numpy(wl.normal(64, 256, seed=2)).cuda(), "euclidean", layout="full")
+    floor = {}
+    for m in ("single", "complete", "average", "ward"):
+        run_rounds(small.clone(), m)
+        floor[m] = min(run_rounds(small.clone(), m) for _ in range(3)) * 1e3 / 63
+    n = 8192
+    x = torch.from_numpy(wl.normal(n, 256, seed=2)).cuda()
+    d0 = gt.dist(x, "euclidean", layout="full")
+    hc = {"reference_floor_ms": round(n ** 3 / 6 * 4 / 8e12 * 1e3, 2)}
+    for m in ("single", "complete", "average", "ward"):
+        ms = run_rounds(d0.clone(), m)
+        us = 1e3 * ms / (n - 1)
+        hc[m] = {"ms": round(ms, 2), "us_per_round": round(us, 2),
+                 "roofline": {"bound": "latency", "achieved": round(us, 2),
+                              "floor": round(floor[m], 2), "unit": "us/round",
+                              "frac": round(floor[m] / us, 4),
+                              "floor_from": "the same kernel at n = 64"}}
     out["hclust_8192"] = hc
 
 
@@ -292,7 +413,8 @@ def path_kendall(torch, gt, wl, steps, warmup, out):
                      "unit": "T int32 lane-ops/s", "frac": round(tops / 39.3, 4)}}
 
 
-PATHS = {"methods": path_methods, "batched": path_batched, "backward": path_backward,
+PATHS = {"methods": path_methods, "binary_large": path_binary_large, "uniform": path_uniform,
+         "batched": path_batched, "backward": path_backward,
          "small": path_small, "hclust": path_hclust, "pmcc": path_pmcc, "kendall": path_kendall}
 
 
@@ -439,17 +561,34 @@ def main():
         from oracle import oracle as orc
         rows = head["_rows"]
         xs = np.ascontiguousarray(xh[rows])
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        host = host_facts()
+        threads = args.cpu_threads or host["threads_available"]
         print("bench: cpu baseline, oracle on %d rows, %d threads" % (rows.size, threads),
               file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ref = orc.dist(xs, "euclidean", nthreads=threads)
         dt = time.perf_counter() - t0
+        # one core, on a smaller sample of the same rows (its pairs are a
+        # subset of the sample above)
+        r1 = min(rows.size, 3000)
+        t1 = time.perf_counter()
+        one = orc.dist(np.ascontiguousarray(xs[:r1]), "euclidean", nthreads=1)
+        dt1 = time.perf_counter() - t1
         line["cpu_baseline"] = {"value": round(ref.size / dt, 1), "unit": "pair-dists/s",
                                 "cores": threads, "kind": "port",
                                 "sample": "all %d pairs among %d seeded rows of the %dx%d "
-                                          "mixture, FP64 oracle, %.2f s"
-                                          % (ref.size, rows.size, n, k, dt)}
+                                          "mixture, FP64 oracle (the restatement of R "
+                                          "stats::dist), %.2f s on %d threads"
+                                          % (ref.size, rows.size, n, k, dt, threads),
+                                "one_core": {"value": round(one.size / dt1, 1), "cores": 1,
+                                             "sample": "all %d pairs among the first %d of "
+                                                       "those rows, %.2f s" % (one.size, r1, dt1)},
+                                "host": host}
+        rtime = time_r_stats_dist(np.ascontiguousarray(xs[:r1]))
+        if rtime is not None:
+            line["cpu_baseline"]["r_stats_dist"] = {
+                "value": round(one.size / rtime, 1), "cores": 1,
+                "sample": "R stats::dist on the same %d rows (FP64), %.2f s" % (r1, rtime)}
         errs = {}
         for precision, rec in results.items():
             g = rec["_sample"].astype(np.float64)
@@ -458,7 +597,7 @@ def main():
         line["max_rel_err_by_path"] = {p: float("%.3g" % e) for p, e in errs.items()}
         line["error_sample_pairs"] = int(ref.size)
 
-    if args.host_path:
+    if not args.no_host_path:
         # The host-buffer boundary (gt_dist_host, the counterpart of distance()):
         # H2D of the points, the kernels, D2H of the packed result into pageable
         # host memory, device allocations included.  Never `value`.
@@ -472,17 +611,54 @@ def main():
             if s != 0:
                 raise RuntimeError(_lib.last_error())
 
-        print("bench: host-buffer path, 1 warm-up + 2 timed calls", file=sys.stderr, flush=True)
+        print("bench: host-buffer path, 1 warm-up + 1 timed call", file=sys.stderr, flush=True)
         host_call()                          # also first-touches the output pages
         t0 = time.perf_counter()
-        for _ in range(2):
-            host_call()
-        hms = (time.perf_counter() - t0) / 2 * 1e3
+        host_call()
+        hms = (time.perf_counter() - t0) * 1e3
+        # the transfers alone, from and to pageable host memory that has been
+        # touched before (the host path's own buffers), as torch copies
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        torch.from_numpy(xh).cuda()
+        torch.cuda.synchronize()
+        h2d = (time.perf_counter() - t0) * 1e3
+        dst = torch.from_numpy(out_h)
+        t0 = time.perf_counter()
+        dst.copy_(out)
+        d2h = (time.perf_counter() - t0) * 1e3
         line["host_path"] = {"ms": round(hms, 2), "pairs_per_s": round(npairs / (hms * 1e-3), 1),
+                             "h2d_points_ms": round(h2d, 2),
+                             "d2h_packed_result_ms": round(d2h, 2),
+                             "d2h_gb_per_s": round(4.0 * npairs / (d2h * 1e-3) / 1e9, 2),
                              "what": "gt_dist_host: H2D of X, kernels, D2H of the %.2f GB packed "
                                      "result to pageable host memory, allocations included"
                                      % (4.0 * npairs / 1e9)}
         del out_h
+
+    if args.exhaustive:
+        # every pair of the workload against the FP64 oracle: the packed result
+        # of one more call per precision, copied to the host and compared by
+        # the oracle's streaming comparator on all host threads
+        from oracle import oracle as orc
+        threads = host_facts()["threads_available"]
+        ex = {}
+        for precision in results:
+            ws_bytes = gt.workspace_size(n, k, "euclidean", precision)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda") if ws_bytes else None
+            time_path(lib, x, out, ws, ws_bytes, precision, 1, 0, torch)
+            del ws
+            got = out.cpu().numpy()
+            print("bench: exhaustive check of %s, %d pairs, %d threads" % (precision, npairs, threads),
+                  file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            err, (wi, wj) = orc.compare_packed(xh, got, "euclidean", nthreads=threads)
+            ex[precision] = {"max_rel_err": float("%.4g" % err), "worst_pair": [int(wi), int(wj)],
+                             "pairs": npairs, "seconds": round(time.perf_counter() - t0, 1)}
+            print("bench: %s max rel err %.3g at (%d, %d)" % (precision, err, wi, wj),
+                  file=sys.stderr, flush=True)
+            del got
+        line["exhaustive_check"] = ex
 
     if not args.no_paths:
         print("bench: secondary paths", file=sys.stderr, flush=True)

@@ -55,6 +55,7 @@ c_cpp = ctypes.POINTER(ctypes.c_char_p)
 SIGNATURES = {
     "Rdistances": (None, [c_fp, c_ip, c_ip, c_fp, c_cpp, c_fp]),
     "Rdistances_packed": (None, [c_fp, c_ip, c_ip, c_fp, c_cpp, c_fp]),
+    "Rdistances_packed_prec": (None, [c_fp, c_ip, c_ip, c_fp, c_cpp, c_fp, c_cpp]),
     "Rdistclust": (None, [c_cpp, c_cpp, c_fp, c_ip, c_ip, c_ip, c_ip, c_fp]),
     "cuCompile": (None, [c_ip, c_cpp, c_cpp]),
     "unloadPackage": (None, []),

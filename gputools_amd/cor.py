@@ -180,12 +180,13 @@ def pmcc(a, b=None, use="everything", precision: str = "bf16x3", stream=None, ou
     return pairs, corr, ts
 
 
-def kendall(a, b=None, stream=None, out=None):
+def kendall(a, b=None, stream=None, out=None, workspace=None):
     """Kendall's tau (the reference's definition: sample pairs tied in both
     variables count as concordant) of the rows of ``a`` (na x n float32 CUDA
     tensor, one variable per row) with the rows of ``b`` (default ``a``).
     Returns an nb x na float64 tensor, entry (y, x) = tau(a[x], b[y]) (the
-    reference's layout), or fills ``out``."""
+    reference's layout), or fills ``out``.  ``workspace``: an optional uint8
+    CUDA tensor to use instead of a fresh one (any contents)."""
     import torch
     for t, name in ((a, "a"), (b, "b")):
         if t is None:
@@ -204,11 +205,17 @@ def kendall(a, b=None, stream=None, out=None):
         raise ValueError("out must be an nb x na float64 tensor with unit column stride")
     nbytes = ctypes.c_size_t()
     check(lib().gt_kendall_workspace_size(na, nb, n, 1 if b is None else 0, ctypes.byref(nbytes)))
-    ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=a.device)
+    if workspace is None:
+        ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=a.device)
+    else:
+        if workspace.dtype != torch.uint8 or not workspace.is_cuda or workspace.numel() < nbytes.value:
+            raise ValueError("workspace must be a uint8 CUDA tensor of at least %d bytes"
+                             % nbytes.value)
+        ws = workspace
     s = stream if stream is not None else torch.cuda.current_stream()
     check(lib().gt_kendall(a.data_ptr(), na, a.stride(0), None if b is None else b.data_ptr(), nb,
                            a.stride(0) if b is None else b.stride(0), n, out.data_ptr(),
-                           out.stride(0), ws.data_ptr(), nbytes.value, s.cuda_stream or None))
+                           out.stride(0), ws.data_ptr(), ws.numel(), s.cuda_stream or None))
     return out
 
 

@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -174,6 +175,42 @@ int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, floa
     return GT_OK;
 }
 
+// Fault injection for tests of the failure paths: with GT_FAIL_ALLOC=N
+// (N >= 1) in the environment, the Nth device allocation the library makes
+// after the variable was last set (or changed) fails as if the device were
+// out of memory.  The reference longjmps out of such a failure and leaks
+// what it had allocated (src/distance.cpp:277-311, src/cuseful.cpp:95-102);
+// here every entry point releases what it holds and returns GT_ERR_ALLOC.
+std::mutex g_fi_mu;
+char g_fi_value[32] = "";
+long g_fi_seen = 0;
+
+bool injected_alloc_failure()
+{
+    const char *v = getenv("GT_FAIL_ALLOC");
+    std::lock_guard<std::mutex> lock(g_fi_mu);
+    if (v == nullptr || *v == '\0') {
+        g_fi_value[0] = '\0';
+        return false;
+    }
+    if (strncmp(v, g_fi_value, sizeof(g_fi_value) - 1) != 0) {
+        strncpy(g_fi_value, v, sizeof(g_fi_value) - 1);
+        g_fi_value[sizeof(g_fi_value) - 1] = '\0';
+        g_fi_seen = 0;
+    }
+    const long n = strtol(v, nullptr, 10);
+    return n >= 1 && ++g_fi_seen == n;
+}
+
+hipError_t device_malloc(void **p, size_t bytes)
+{
+    if (injected_alloc_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
 // Stream-ordered scratch: hipMallocAsync / hipFreeAsync on the caller's stream
 // (safe against concurrent calls on other streams).  Falls back to a
 // synchronous hipMalloc/hipFree pair only where the stream-ordered pool is
@@ -190,7 +227,7 @@ struct Scratch {
     {
         stream = s;
         if (bytes == 0) return GT_OK;
-        hipError_t e = hipMallocAsync(&ptr, bytes, s);
+        hipError_t e = injected_alloc_failure() ? hipErrorOutOfMemory : hipMallocAsync(&ptr, bytes, s);
         if (e == hipSuccess) {
             async = true;
             return GT_OK;
@@ -204,7 +241,7 @@ struct Scratch {
                         "pass a workspace (gt_dist_ws) or call gt_dist_noalloc",
                         hipGetErrorString(e));
         if (e != hipErrorNotSupported) return hip_fail(e, "hipMallocAsync(workspace)");
-        e = hipMalloc(&ptr, bytes);
+        e = device_malloc(&ptr, bytes);
         if (e != hipSuccess) {
             ptr = nullptr;
             return hip_fail(e, "hipMalloc(workspace)");
@@ -259,7 +296,7 @@ struct DeviceBuffer {
     int alloc(size_t bytes, const char *what)
     {
         if (bytes == 0) return GT_OK;
-        hipError_t e = hipMalloc(&ptr, bytes);
+        hipError_t e = device_malloc(&ptr, bytes);
         if (e != hipSuccess) {
             ptr = nullptr;
             return hip_fail(e, what);
@@ -695,7 +732,9 @@ int gt_dist_device_full(const float *x_host, int64_t n, int64_t k, int method, f
     if ((s = dx.alloc((size_t)n * (size_t)k * sizeof(float), "hipMalloc(points)")) != GT_OK) return s;
     if ((s = dws.alloc(ws_bytes, "hipMalloc(workspace)")) != GT_OK) return s;
     // Pitched n x n result, as distanceLeaveOnGpu allocates it (cudaMallocPitch).
-    hipError_t e = hipMallocPitch(&d, &pitch, (size_t)n * sizeof(float), (size_t)n);
+    hipError_t e = injected_alloc_failure()
+                       ? hipErrorOutOfMemory
+                       : hipMallocPitch(&d, &pitch, (size_t)n * sizeof(float), (size_t)n);
     if (e != hipSuccess) return hip_fail(e, "hipMallocPitch(distances)");
     if (k > 0) {
         e = hipMemcpy(dx.ptr, x_host, (size_t)n * (size_t)k * sizeof(float), hipMemcpyHostToDevice);
@@ -907,6 +946,29 @@ void Rdistances_packed(const float *points, const int *numPoints, const int *dim
     } else {
         const int m = dist_method_from_name(*method);
         s = dist_host(points, *numPoints, *dim, m, *p, GT_PREC_FP32, GT_LAYOUT_PACKED, distances);
+    }
+    raise_in_r(s);
+}
+
+void Rdistances_packed_prec(const float *points, const int *numPoints, const int *dim,
+                            float *distances, const char **method, const float *p,
+                            const char **precision)
+{
+    g_err[0] = '\0';
+    int s;
+    if (!numPoints || !dim || !method || !p || !precision || !*precision) {
+        s = fail(GT_ERR_INVALID, "Rdistances_packed_prec: null argument");
+    } else {
+        int prec = -1;
+        if (strcmp(*precision, "fp32") == 0) prec = GT_PREC_FP32;
+        else if (strcmp(*precision, "bf16x3") == 0) prec = GT_PREC_BF16X3;
+        else if (strcmp(*precision, "fp16x3") == 0) prec = GT_PREC_FP16X3;
+        if (prec < 0) {
+            s = fail(GT_ERR_INVALID, "unknown precision \"%s\" (fp32, bf16x3 or fp16x3)", *precision);
+        } else {
+            const int m = dist_method_from_name(*method);
+            s = dist_host(points, *numPoints, *dim, m, *p, prec, GT_LAYOUT_PACKED, distances);
+        }
     }
     raise_in_r(s);
 }

@@ -56,39 +56,32 @@ __host__ __device__ __forceinline__ int64_t words_upto(int64_t m)
     return 16 * q * (q + 1) + r * (q + 1);
 }
 
-// The six words of one step into lanes L and L + 1 of the plane words:
-// v_writelane_b32 with the lane select as an inline constant.  The values
-// are compare results just written to SGPRs by the VALU; the compiler does
-// not see inside the asm, so the s_nop supplies the wait states between
-// that write and the lane writes that read it.
+// v_writelane_b32 through the compiler's own intrinsic (this clang has no
+// builtin for it), so that the compiler sees the instruction and places the
+// wait states between the compare that writes the ballot's SGPRs and the
+// lane write that reads them.
+extern "C" __device__ int gt_amdgcn_writelane(int value, int lane, int old)
+    __asm("llvm.amdgcn.writelane.i32");
+
+// One plane's two words of one step (a 64-bit ballot) into lanes L and
+// L + 1 of the plane words: v_writelane_b32 with the lane select as an
+// inline constant.
+template <int L>
+__device__ __forceinline__ void put_plane(uint32_t &w, unsigned long long bits)
+{
+    w = (uint32_t)gt_amdgcn_writelane((int)(uint32_t)bits, L, (int)w);
+    w = (uint32_t)gt_amdgcn_writelane((int)(uint32_t)(bits >> 32), L + 1, (int)w);
+}
+
+// The three planes' words of one step.
 template <int L>
 __device__ __forceinline__ void put_step(uint32_t &wp, uint32_t &wn, uint32_t &we,
                                          unsigned long long gt, unsigned long long lt,
                                          unsigned long long eq)
 {
-    asm volatile("s_nop 4\n\t"
-                 "v_writelane_b32 %0, %3, %9\n\t"
-                 "v_writelane_b32 %0, %4, %10\n\t"
-                 "v_writelane_b32 %1, %5, %9\n\t"
-                 "v_writelane_b32 %1, %6, %10\n\t"
-                 "v_writelane_b32 %2, %7, %9\n\t"
-                 "v_writelane_b32 %2, %8, %10"
-                 : "+v"(wp), "+v"(wn), "+v"(we)
-                 : "s"((uint32_t)gt), "s"((uint32_t)(gt >> 32)), "s"((uint32_t)lt),
-                   "s"((uint32_t)(lt >> 32)), "s"((uint32_t)eq), "s"((uint32_t)(eq >> 32)),
-                   "i"(L), "i"(L + 1));
-}
-
-// One plane's two words of one step into lanes L and L + 1 (the same wait
-// states as put_step).
-template <int L>
-__device__ __forceinline__ void put_plane(uint32_t &w, unsigned long long bits)
-{
-    asm volatile("s_nop 4\n\t"
-                 "v_writelane_b32 %0, %1, %3\n\t"
-                 "v_writelane_b32 %0, %2, %4"
-                 : "+v"(w)
-                 : "s"((uint32_t)bits), "s"((uint32_t)(bits >> 32)), "i"(L), "i"(L + 1));
+    put_plane<L>(wp, gt);
+    put_plane<L>(wn, lt);
+    put_plane<L>(we, eq);
 }
 
 // f(std::integral_constant<int, U>) for U = 0 .. N - 1, in order.

@@ -43,6 +43,7 @@ struct PlainArgs {
     int64_t tile_offset;  // first logical tile of this launch
     int64_t tiles_this;   // tiles in this launch
     float p, inv_p, fk;
+    int fold;             // chunks summed in `part` before it is added to `acc`
     bool vec;             // ldd and out allow 16-byte stores (pitched layouts)
 };
 
@@ -185,6 +186,7 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
     int clean_terms = 0;                     // canberra: features of CB_CLEAN / CB_ZEROS chunks
 
     gload(0);
+    int folded = 0;   // chunks in `part` since the last fold
     for (int64_t kb = 0; kb < k; kb += PBK) {
         __syncthreads();
         bool unclean = false, unranged = false, nonfin = false;
@@ -250,7 +252,10 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
         } else {
             chunk(std::integral_constant<int, CB_IEEE>{});
         }
-        if (OP != OP_MAXIMUM) {
+        // Two-level sum: `part` collects `fold` chunks (about sqrt(k) features),
+        // then joins the total, so both levels add about sqrt(k / 16) terms.
+        if (OP != OP_MAXIMUM && (++folded == args.fold || kb + PBK >= k)) {
+            folded = 0;
 #pragma unroll
             for (int r = 0; r < 8; ++r)
 #pragma unroll
@@ -367,6 +372,14 @@ hipError_t launch_plain(int op, int layout, const float *a, int64_t na, int64_t 
     args.p = p;
     args.inv_p = 1.0f / p;
     args.fk = (float)k;
+    {
+        // chunks per fold: about sqrt(chunks), so that the partial and the
+        // total each add about sqrt(k / 16) values
+        const int64_t chunks = (k + PBK - 1) / PBK;
+        int f = 1;
+        while ((int64_t)(f + 1) * (f + 1) <= chunks) ++f;
+        args.fold = f;
+    }
     const int64_t ta = (na + PBM - 1) / PBM, tb = (nb + PBM - 1) / PBM;
     int64_t total;
     if (layout == OUT_CROSS) {

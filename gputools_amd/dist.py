@@ -125,23 +125,27 @@ def _as_points(points) -> tuple:
     return x, labels
 
 
-def gpuDist(points, method: str = "euclidean", p: float = 2.0) -> Dist:
+def gpuDist(points, method: str = "euclidean", p: float = 2.0, precision: str = "fp32") -> Dist:
     """Distances between the rows of ``points`` (R/gpuHclust.R:1-39).
 
     ``method`` is one of "euclidean", "maximum", "manhattan", "canberra",
     "binary", "minkowski" (unique prefixes accepted).  ``p`` is the Minkowski
-    power.  Returns a :class:`Dist` whose values are the FP32 GPU results
-    widened to double.
+    power.  ``precision`` (not in the reference, whose gpuDist has one FP32
+    path) selects "fp32" or, for euclidean, the split-precision Gram form
+    "bf16x3" / "fp16x3" (max relative error 1e-5).  Returns a :class:`Dist`
+    whose values are the FP32 GPU results widened to double.
     """
     call = "gpuDist(points = points, method = %r, p = %r)" % (method, p)
+    if precision not in _lib.PRECISIONS:
+        raise ValueError("precision must be one of %s" % ", ".join(_lib.PRECISIONS))
     method = match_method(method)
     x, labels = _as_points(points)
     n, k = x.shape
     p32 = float(np.float32(p))
     out = np.empty(max(n * (n - 1) // 2, 0), dtype=np.float32)
     m = METHODS.index(method)
-    check(lib().gt_dist_host(x.ctypes.data, n, k, m, p32, _lib.PREC_FP32, _lib.LAYOUT_PACKED,
-                             out.ctypes.data if out.size else None))
+    check(lib().gt_dist_host(x.ctypes.data, n, k, m, p32, _lib.PRECISIONS[precision],
+                             _lib.LAYOUT_PACKED, out.ctypes.data if out.size else None))
     return Dist(out.astype(np.float64), n, labels=labels, method=method, call=call,
                 p=p if method == "minkowski" else None)
 
@@ -176,6 +180,25 @@ def Rdistances_packed(points: np.ndarray, method: str, p: float) -> np.ndarray:
     fp = ctypes.POINTER(ctypes.c_float)
     lib().Rdistances_packed(x.ctypes.data_as(fp), ctypes.byref(cn), ctypes.byref(ck),
                             d.ctypes.data_as(fp), cm, ctypes.byref(cp))
+    err = _lib.last_error()
+    if err:
+        raise _lib.GputoolsError(_lib.GT_ERR_HIP, err)
+    return d[:n * (n - 1) // 2]
+
+
+def Rdistances_packed_prec(points: np.ndarray, method: str, p: float, precision: str) -> np.ndarray:
+    """The packed .C export with a precision string, called as an R shim
+    would (every argument a pointer, strings as char**)."""
+    x = np.ascontiguousarray(points, dtype=np.float32)
+    n, k = x.shape
+    d = np.zeros(max(n * (n - 1) // 2, 1), dtype=np.float32)
+    cn, ck = ctypes.c_int(n), ctypes.c_int(k)
+    cm = (ctypes.c_char_p * 1)(method.encode())
+    cprec = (ctypes.c_char_p * 1)(precision.encode())
+    cp = ctypes.c_float(p)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib().Rdistances_packed_prec(x.ctypes.data_as(fp), ctypes.byref(cn), ctypes.byref(ck),
+                                 d.ctypes.data_as(fp), cm, ctypes.byref(cp), cprec)
     err = _lib.last_error()
     if err:
         raise _lib.GputoolsError(_lib.GT_ERR_HIP, err)

@@ -118,6 +118,16 @@ void Rdistances(const float *points, const int *numPoints, const int *dim,
 void Rdistances_packed(const float *points, const int *numPoints, const int *dim,
                        float *distances, const char **method, const float *p);
 
+/* Rdistances_packed with a choice of arithmetic for R callers: *precision is
+ * "fp32" (the FP32 difference kernels, what Rdistances computes), "bf16x3"
+ * or "fp16x3" (the split-precision Gram form, euclidean only, max relative
+ * error <= 1e-5 against FP64).  No counterpart in the reference, whose
+ * gpuDist (R/gpuHclust.R:21-27) has one FP32 path; an unchanged gpuDist
+ * keeps calling Rdistances.  Unknown precision names are an error. */
+void Rdistances_packed_prec(const float *points, const int *numPoints, const int *dim,
+                            float *distances, const char **method, const float *p,
+                            const char **precision);
+
 /* Replaces Rdistclust (src/rinterface.cpp:172-204, decl src/rinterface.h:50-52).
  * Distance stage in the device-full layout with p = 2 (src/rinterface.cpp:182),
  * then the clustering stage on that matrix (gt_hclust, lambda = beta = 0.5),

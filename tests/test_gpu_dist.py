@@ -95,6 +95,33 @@ def test_ragged_sizes(n, k, method):
     assert_parity(run(x, method, p), orc.dist(x, method, p), method)
 
 
+@pytest.mark.parametrize("n,k", [(300, 4096), (130, 16384), (64, 65536), (200, 3001)])
+@pytest.mark.parametrize("method", ["euclidean", "maximum", "manhattan", "canberra", "binary",
+                                    "minkowski"])
+def test_many_features(n, k, method):
+    """Thousands of features: the chunk partials are summed in two balanced
+    levels (about sqrt(k) features each), which keeps the FP32 sum within
+    the 1e-5 bar at k = 65536.  Every pair against the FP64 oracle."""
+    x = wl.method_input(method, n, k, seed=n + k)
+    if method not in ("canberra", "binary"):
+        x = x + np.float32(0.25)          # a mean, so that sums do not cancel to zero
+    p = 3.0 if method == "minkowski" else 2.0
+    print("case many features", method, n, k, flush=True)
+    assert_parity(run(x, method, p), orc.dist(x, method, p), method)
+
+
+@pytest.mark.parametrize("method", ["euclidean", "manhattan", "minkowski"])
+def test_many_features_minkowski_powers_and_cross(method):
+    x = wl.uniform(100, 20000, seed=19)
+    y = wl.uniform(37, 20000, seed=20)
+    p = 1.5 if method == "minkowski" else 2.0
+    got = gt.dist_cross(dev(x), dev(y), method, p)
+    torch.cuda.synchronize()
+    want = orc.dist_cross(x, y, method, p)
+    err = np.max(np.abs(got.cpu().numpy().astype(np.float64) - want) / want)
+    assert err <= (2e-5 if method == "minkowski" else RTOL), err
+
+
 @pytest.mark.parametrize("method", ["euclidean", "maximum", "manhattan", "canberra", "binary",
                                     "minkowski"])
 def test_full_layout_matches_packed_with_pitch(method):
@@ -232,6 +259,43 @@ def test_Rdistances_packed_dotC_convention():
         xin = np.abs(x) + 0.1 if method == "canberra" else x
         got = gt.Rdistances_packed(xin, method, p)
         assert_parity(got, orc.dist(xin, method, p), method)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "fp32"])
+def test_Rdistances_packed_prec_dotC_convention(precision):
+    """The precision-selecting .C export, called with R's pointer convention."""
+    x = wl.mixture(1500, 128, seed=23) + np.float32(3.0)
+    got = gt.Rdistances_packed_prec(x, "euclidean", 2.0, precision)
+    assert_parity(got, orc.dist(x, "euclidean"), "euclidean")
+    d = gt.gpuDist(x, "euclidean", precision=precision)
+    np.testing.assert_array_equal(d.values, got.astype(np.float64))
+
+
+def test_Rdistances_packed_prec_errors():
+    x = wl.normal(20, 4, seed=24)
+    with pytest.raises(gt.GputoolsError, match="euclidean only"):
+        gt.Rdistances_packed_prec(x, "manhattan", 2.0, "bf16x3")
+    with pytest.raises(gt.GputoolsError, match="unknown precision"):
+        gt.Rdistances_packed_prec(x, "euclidean", 2.0, "fp8")
+    # fp32 takes every method, as Rdistances does
+    got = gt.Rdistances_packed_prec(np.abs(x) + 0.1, "canberra", 2.0, "fp32")
+    assert_parity(got, orc.dist(np.abs(x) + 0.1, "canberra"), "canberra")
+
+
+def test_device_selection_round_trip():
+    """chooseGpu / getGpuId through rsetDevice / rgetDevice
+    (src/correlation.cpp:163-175): a valid id becomes current, ids outside
+    0 .. count-1 fail with the reference's message."""
+    count = torch.cuda.device_count()
+    assert gt.chooseGpu(0) == [0]
+    assert gt.getGpuId() == 0
+    assert gt.chooseGpu(0.9) == [0]          # as.integer truncates
+    for bad in (-1, count, count + 5):
+        with pytest.raises(gt.GputoolsError, match="The gpu id number is not valid."):
+            gt.chooseGpu(bad)
+    assert gt.getGpuId() == 0                # a rejected id leaves the device alone
+    x = wl.normal(50, 6, seed=25)             # and the library still computes
+    assert_parity(gt.gpuDist(x).values, orc.dist(x, "euclidean"), "euclidean")
 
 
 def test_gpuDist_mirror():

@@ -142,3 +142,37 @@ def test_kendall_long_variables_global_rows():
     torch.cuda.synchronize()
     want, _ = orc.kendall(a, b, nthreads=8)
     assert_same(tau.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("same", [True, False])
+def test_kendall_workspace_with_garbage_reused(same):
+    """One workspace filled with 0xFF bytes, then used for two calls on
+    different data: every word the count reads (the P and N planes up to the
+    padded row length, the E planes, the flags, the counts) must be written
+    or cleared by each call, whatever the workspace held."""
+    rng = np.random.default_rng(71 if same else 72)
+    cases = []
+    # n = 100: 204 words of sample pairs, padded further for the chunks
+    x1 = rng.standard_normal((150, 100)).astype(np.float32)            # tie-free
+    x2 = rng.integers(0, 4, size=(90, 100)).astype(np.float32)         # ties everywhere
+    x2[rng.random(x2.shape) < 0.05] = np.nan                           # and NaN
+    for xa in (x1, x2):
+        xb = None if same else rng.standard_normal((70, 100)).astype(np.float32)
+        cases.append((xa, xb))
+    need = max(gt_kendall_ws_bytes(xa, xb) for xa, xb in cases)
+    ws = torch.full((need + 4096,), 0xFF, dtype=torch.uint8, device="cuda")
+    for xa, xb in cases:
+        a = torch.from_numpy(xa).cuda()
+        b = None if xb is None else torch.from_numpy(xb).cuda()
+        got = gt.kendall(a, b, workspace=ws).cpu().numpy()
+        want, _ = orc.kendall(xa, xa if xb is None else xb, nthreads=8)
+        assert_same(got, want)
+
+
+def gt_kendall_ws_bytes(xa, xb):
+    import ctypes
+    nb = ctypes.c_size_t()
+    na, n = xa.shape
+    gt.lib().gt_kendall_workspace_size(na, na if xb is None else xb.shape[0], n,
+                                       1 if xb is None else 0, ctypes.byref(nb))
+    return nb.value
