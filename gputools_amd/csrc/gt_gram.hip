@@ -37,56 +37,44 @@
 namespace gt {
 namespace {
 
-constexpr int GBM = 256;        // B-side tile rows (i); the A side (j) has WA * 64
+constexpr int GBM = 256;        // tile side (points)
 constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
 constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
-constexpr int GSUPER = 4;       // super-tile side in 256-row tiles (L2 locality)
-constexpr int FBK = 32;         // features per fix-up chunk
-constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
-constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
+constexpr int GTHREADS = 1024;  // 16 waves in a 4 (A) x 4 (B) grid, each 64 x 64 pairs
+constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
+constexpr int FBK = 32;         // features per serial fix-up chunk
+constexpr int FLD = FBK + 4;    // serial fix-up LDS row length (floats, 16 B aligned rows)
+constexpr int FCAP = 1024;      // flagged pairs per serial fix-up pass
 // Gram-form values are finished with the hardware square root (v_sqrt_f32)
 // plus one residual correction (sqrt_rn), which rounds correctly for normal
 // inputs; d^2 below 2^-100 goes to the exact pass instead.
 constexpr float GRAM_SQRT_MIN = 0x1p-100f;
+constexpr int FPER = FCAP / GTHREADS;
+constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
+constexpr int TERM_BYTES = GBM * GROWB;        // one split term of one side, one chunk (8 KB)
+constexpr int FIX_BYTES = 2 * GBM * FLD * 4;   // serial fix-up staging, both sides
 constexpr int LIST_BYTES = FCAP * 4;
+constexpr int SIDE_BYTES = 4 * GBM * 4 + 16 * 4;  // norms A, B, inverse scales A, B, scan
 
-// Tile shape: WA x 4 waves, each 64 x 64 pairs, so WA * 64 A-side rows (j)
-// against 256 B-side rows (i).  WA = 4: one 1024-thread workgroup per CU and
-// a three-stage ring.  WA = 2: 512 threads, a 128 x 256 half of a 256 x 256
-// tile, two workgroups per CU (each workgroup's epilogue and exact pass
-// overlap the other's MFMA loop), two stages for three split terms.
-// LDS: [main-loop split terms, NSTAGE stages | (aliased) fix-up staging +
-// pair list] [norms A, norms B, inverse scales A, B, scan]
-template <int NT, int WA>
-struct GramCfg {
-    static constexpr int BA = WA * 64;
-    static constexpr int BB = GBM;
-    static constexpr int NW = WA * 4;
-    static constexpr int THREADS = NW * 64;
-    static constexpr int HALVES = 4 / WA;           // A-side parts of a 256 x 256 tile
-    static constexpr int NG = (BA + BB) / 32;       // 32-row load groups
-    static constexpr int TERMA = BA * GROWB;        // one term, A side, one chunk
-    static constexpr int TERMB = BB * GROWB;
-    static constexpr int STAGE = NT * (TERMA + TERMB);
-    static constexpr int NSTAGE = (WA == 4 || NT == 2) ? 3 : 2;
-    static constexpr int MAIN = NSTAGE * STAGE;
-    static constexpr int FIXB = (BA + BB) * FLD * 4;
-    static constexpr int FIX = FIXB + LIST_BYTES;
+// LDS: [split terms, three stages | (aliased) fix-up staging + pair list]
+// [side data]
+template <int NT>
+struct TileLds {
+    static constexpr int STAGE = 2 * NT * TERM_BYTES;
+    static constexpr int MAIN = 3 * STAGE;
+    static constexpr int FIX = FIX_BYTES + LIST_BYTES;
     static constexpr int REGION = MAIN > FIX ? MAIN : FIX;
-    static constexpr int BYTES = REGION + 2 * (BA + BB) * 4 + 16 * 4;
-    static constexpr int FUNITS = (BA + BB) * FBK / 4 / THREADS;  // float4 staging units per thread
-    static constexpr int FPER = FCAP / THREADS;
-    static_assert((BA + BB) * FBK / 4 % THREADS == 0, "staging units");
-    static_assert(FCAP % THREADS == 0, "list");
+    static constexpr int SIDE = REGION;
+    static constexpr int BYTES = REGION + SIDE_BYTES;
 };
 
 struct GramArgs {
-    const float *x;        // FP32 points (fix-up)
+    const float *x;        // FP32 points (exact passes)
     int64_t n, k, ldx;
     const unsigned short *terms;  // NT x n x kp split terms
     int64_t kp;            // k padded to KPAD
-    const float *norms;    // ||x_i||^2 (FP32)
+    const float *norms;    // ||x_i - c||^2 (FP32)
     const float *iscale;   // FP16 path: inverse power-of-two row scale
     float *out;
     int64_t ldd;
@@ -131,8 +119,7 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     }
 }
 
-// Exclusive scan of one int per thread over a workgroup of NW waves.
-template <int NW>
+// Exclusive scan of one int per thread over the 1024-thread workgroup.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -147,7 +134,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
     int before = 0;
     total = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
+    for (int w = 0; w < GTHREADS / 64; ++w) {
         const int c = tmp[w];
         if (w < wave) before += c;
         total += c;
@@ -163,172 +150,109 @@ __device__ __forceinline__ void wait_vmcnt()
     asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-template <int SPLIT, int LAYOUT, bool VEC4, int WA>
-__global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
-    GramArgs args)
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void gbl_void_t;
+
+// Main-loop LDS image of one chunk: side A (rows j) terms 0..NT-1, then side
+// B (rows i), each 256 rows x 32 B; the two 16 B halves of row R are swapped
+// when swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
+// 8 or 16 consecutive rows hit distinct banks without padding.
+// global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so each
+// wave instruction fills 32 rows of one term: wave w fills rows
+// 32 (w % 8) .. + 31 of side w / 8.  Rows past n are clamped to row n - 1:
+// they only feed pairs that are never stored.
+__device__ __forceinline__ const unsigned short *chunk_source(const GramArgs &args, int64_t i0,
+                                                              int64_t j0, int wave, int lane)
+{
+    const int row = (wave & 7) * 32 + (lane >> 1);
+    const int half = (lane & 1) ^ (((row >> 2) ^ (row >> 3)) & 1);
+    const int64_t r = ((wave >> 3) ? i0 : j0) + row;
+    return args.terms + (r < args.n ? r : args.n - 1) * args.kp + half * 8;
+}
+
+template <int NT>
+__device__ __forceinline__ void issue_chunk(const GramArgs &args, const unsigned short *gsrc,
+                                            unsigned char *stage, int wave, int chunk)
+{
+    const int64_t term_stride = args.n * args.kp;
+    const int64_t kb = (int64_t)chunk * GBK;
+    unsigned char *dst = stage + (wave >> 3) * NT * TERM_BYTES + (wave & 7) * 1024;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc + t * term_stride + kb),
+                                         (lds_void_t *)(dst + t * TERM_BYTES), 16, 0, 0);
+}
+
+// One chunk of MFMA products for this wave's 64 x 64 pairs (acc[a][b]: A
+// rows wr*64 + a*32.., B rows wc*64 + b*32..); sg = stage + the lane's
+// operand offset.
+template <int SPLIT>
+__device__ __forceinline__ void mfma_chunk(f32x16_t (&acc)[2][2], const unsigned char *sg, int wr,
+                                           int wc)
 {
     typedef typename Split<SPLIT>::vec vec;
     constexpr int NT = Split<SPLIT>::NT;
-    typedef GramCfg<NT, WA> C;
-    constexpr int BA = C::BA, BB = C::BB, THREADS = C::THREADS;
-    __shared__ __attribute__((aligned(1024))) unsigned char smem[C::BYTES];
-    float *fS = reinterpret_cast<float *>(smem);          // fix-up staging (aliases the terms)
-    unsigned int *list = reinterpret_cast<unsigned int *>(smem + C::FIXB);
-    float *nA = reinterpret_cast<float *>(smem + C::REGION);  // row norms, A side (j)
-    float *nB = nA + BA;                                  // B side (i)
-    float *iA = nB + BB;                                  // inverse row scales (FP16 path)
-    float *iB = iA + BA;
-    int *scan_tmp = reinterpret_cast<int *>(iB + BB);
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave >> 2, wc = wave & 3;   // A rows wr*64.., B rows wc*64..
-
-    // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
-    // GSUPER tiles of 256 x 256, each cut into HALVES parts along j; tiles
-    // above the diagonal or past the edge exit at once.
-    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
-    constexpr int PER_SUPER = GSUPER * GSUPER * C::HALVES;
-    const int64_t st = id / PER_SUPER;
-    const int w = (int)(id % PER_SUPER);
-    const int u = w / C::HALVES, h = w % C::HALVES;
-    int SI, SJ;
-    tri_tile(st, SI, SJ);
-    const int64_t I = (int64_t)SI * GSUPER + u / GSUPER;
-    const int64_t J = (int64_t)SJ * GSUPER + u % GSUPER;
-    if (I >= args.T || J > I) return;
-    const int64_t i0 = I * BB, j0 = J * GBM + (int64_t)h * BA;
-    const int64_t n = args.n, k = args.k, kp = args.kp;
-    if (j0 >= n) return;
-
-    if (tid < BA) {
-        const int64_t r = j0 + tid;
-        nA[tid] = r < n ? args.norms[r] : 0.0f;
-        if (SPLIT == SPLIT_FP16) iA[tid] = r < n ? args.iscale[r] : 1.0f;
-    } else if (tid < BA + BB) {
-        const int64_t r = i0 + (tid - BA);
-        nB[tid - BA] = r < n ? args.norms[r] : 0.0f;
-        if (SPLIT == SPLIT_FP16) iB[tid - BA] = r < n ? args.iscale[r] : 1.0f;
-    }
-
-    // ---- main loop: direct global -> LDS loads, NSTAGE-deep, MFMA products -
-    // LDS chunk image: for stage g, side A (rows j) then side B (rows i), term
-    // t, rows x 32 B; the two 16 B halves of row R are swapped when
-    // swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
-    // 8 or 16 consecutive rows hit distinct banks without padding.
-    // global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so
-    // each wave instruction fills 32 rows of one term.  The (BA + BB) / 32 row
-    // groups go to the waves round robin (group q*NW + wave).  Rows past n are
-    // clamped to row n-1: they only feed pairs that are never stored.
-    typedef __attribute__((address_space(3))) void lds_void_t;
-    typedef const __attribute__((address_space(1))) void gbl_void_t;
-    const int64_t term_stride = n * kp;
-    constexpr int GPW = (C::NG + C::NW - 1) / C::NW;   // row groups per wave, at most
-    const int ngw = (wave + C::NW < C::NG) ? 2 : 1;     // this wave's (GPW <= 2)
-    static_assert(GPW <= 2, "load groups");
-    const unsigned short *gsrc[GPW];
-    int ldst[GPW];          // LDS offset of the group's term-0 block within a stage
-    int tstride[GPW];       // bytes between the group's term blocks
-    {
-        const int rr = lane >> 1;
-        const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
+    vec av[NT][2];
 #pragma unroll
-        for (int q = 0; q < GPW; ++q) {
-            const int g = wave + q * C::NW;
-            const bool sideA = g < BA / 32;
-            const int grow = sideA ? g * 32 : (g - BA / 32) * 32;
-            const int64_t r = (sideA ? j0 : i0) + grow + rr;
-            gsrc[q] = args.terms + (r < n ? r : n - 1) * kp + half * 8;
-            ldst[q] = sideA ? grow * GROWB : NT * C::TERMA + grow * GROWB;
-            tstride[q] = sideA ? C::TERMA : C::TERMB;
-        }
-    }
-    auto issue = [&](int chunk) {
-        const int g = chunk % C::NSTAGE;
-        const int64_t kb = (int64_t)chunk * GBK;
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int q = 0; q < GPW; ++q) {
-            if (q < ngw) {
+        for (int a = 0; a < 2; ++a)
+            av[t][a] = *reinterpret_cast<const vec *>(sg + t * TERM_BYTES + (wr * 64 + a * 32) * GROWB);
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    __builtin_amdgcn_global_load_lds(
-                        (gbl_void_t *)(gsrc[q] + t * term_stride + kb),
-                        (lds_void_t *)(smem + g * C::STAGE + ldst[q] + t * tstride[q]), 16, 0,
-                        0);
-            }
-        }
-    };
-    // operand read offset of this lane (rows differ by multiples of 32 between
-    // blocks, which leaves swz unchanged)
-    const int lrow = lane & 31;
-    const int roff = lrow * GROWB + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
-
-    f32x16_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-
-    const int nchunks = (int)(kp / GBK);
-    // prologue: chunks 0 .. NSTAGE-2 in flight
-#pragma unroll
-    for (int c = 0; c < C::NSTAGE - 1; ++c)
-        if (c < nchunks) issue(c);
-    for (int c = 0; c < nchunks; ++c) {
-        // chunk c landed when at most the later chunks already issued are
-        // outstanding (vector memory loads complete in order)
-        if (C::NSTAGE == 3 && c + 1 < nchunks) {
-            if (ngw == 2)
-                wait_vmcnt<2 * NT>();
-            else
-                wait_vmcnt<NT>();
-        } else {
-            wait_vmcnt<0>();
-        }
-        // every wave's part of chunk c is in LDS, and every wave finished
-        // chunk c-1, whose stage the next issue overwrites.  A bare s_barrier:
-        // __syncthreads() would also drain the later chunks' loads.
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-        if (c + C::NSTAGE - 1 < nchunks) issue(c + C::NSTAGE - 1);
-        const unsigned char *sg = smem + (c % C::NSTAGE) * C::STAGE + roff;
-        vec av[NT][2];
+    for (int b = 0; b < 2; ++b) {
+        vec bv[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
+            bv[t] = *reinterpret_cast<const vec *>(sg + (NT + t) * TERM_BYTES +
+                                                   (wc * 64 + b * 32) * GROWB);
+        // hh' first, then the cross terms in decreasing magnitude
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-                av[t][a] = *reinterpret_cast<const vec *>(sg + t * C::TERMA +
-                                                          (wr * 64 + a * 32) * GROWB);
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            vec bv[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                bv[t] = *reinterpret_cast<const vec *>(sg + NT * C::TERMA + t * C::TERMB +
-                                                       (wc * 64 + b * 32) * GROWB);
-            // hh' first, then the cross terms in decreasing magnitude
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[0], acc[a][b]);
-                acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[1], acc[a][b]);
-                acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[0], acc[a][b]);
-                if (NT == 3) {
-                    acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[NT - 1], acc[a][b]);
-                    acc[a][b] = Split<SPLIT>::mfma(av[NT - 1][a], bv[0], acc[a][b]);
-                    acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[1], acc[a][b]);
-                }
+        for (int a = 0; a < 2; ++a) {
+            acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[0], acc[a][b]);
+            acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[1], acc[a][b]);
+            acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[0], acc[a][b]);
+            if (NT == 3) {
+                acc[a][b] = Split<SPLIT>::mfma(av[0][a], bv[NT - 1], acc[a][b]);
+                acc[a][b] = Split<SPLIT>::mfma(av[NT - 1][a], bv[0], acc[a][b]);
+                acc[a][b] = Split<SPLIT>::mfma(av[1][a], bv[1], acc[a][b]);
             }
         }
     }
-    // the fix-up below reuses the stage memory
-    __syncthreads();
+}
 
-    // ---- epilogue: Gram-form distance, or flag for exact recompute ---------
-    // 32x32 accumulator layout: element r of lane l is row 8(r/4) + 4(l/32) +
-    // r%4 (A side, j) and column l%32 (B side, i); lanes 0..31 hold 32
-    // consecutive i of one j, so packed stores are contiguous runs.
-    const bool interior = (I > J) && (i0 + BB <= n);   // every pair valid, i > j
+// Norms (and FP16 inverse row scales) of the tile's rows into LDS.
+template <int SPLIT>
+__device__ __forceinline__ void load_side(const GramArgs &args, int64_t i0, int64_t j0, float *nA,
+                                          float *nB, float *iA, float *iB)
+{
+    const int tid = threadIdx.x;
+    if (tid < GBM) {
+        const int64_t r = j0 + tid;
+        nA[tid] = r < args.n ? args.norms[r] : 0.0f;
+        if (SPLIT == SPLIT_FP16) iA[tid] = r < args.n ? args.iscale[r] : 1.0f;
+    } else if (tid < 2 * GBM) {
+        const int64_t r = i0 + (tid - GBM);
+        nB[tid - GBM] = r < args.n ? args.norms[r] : 0.0f;
+        if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < args.n ? args.iscale[r] : 1.0f;
+    }
+}
+
+// Epilogue of one tile: d^2 = ||x||^2 + ||y||^2 - 2g for this wave's pairs;
+// every valid pair that is not flagged (d^2 >= tau (||x||^2 + ||y||^2)) is
+// stored.  Returns the lane's flag bits: bit (2a + b) * 16 + r marks
+// accumulator element r of block (a, b).  32x32 accumulator layout: element
+// r of lane l is row 8(r/4) + 4(l/32) + r%4 (A side, j) and column l%32 (B
+// side, i); lanes 0..31 hold 32 consecutive i of one j, so packed stores are
+// contiguous runs.
+template <int SPLIT, int LAYOUT>
+__device__ __forceinline__ unsigned long long gram_epilogue(const f32x16_t (&acc)[2][2],
+                                                            const GramArgs &args, int64_t i0,
+                                                            int64_t j0, bool interior, int wr,
+                                                            int wc, int lane, const float *nA,
+                                                            const float *nB, const float *iA,
+                                                            const float *iB)
+{
+    const int64_t n = args.n;
     const int li0 = wc * 64 + (lane & 31);
     float ni[2], sb[2];
 #pragma unroll
@@ -337,12 +261,12 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
         sb[b] = (SPLIT == SPLIT_FP16) ? iB[li0 + b * 32] : 1.0f;
     }
     const float tau = args.tau;
-    // flag bit (2a + b) * 16 + r marks accumulator element r of block (a, b)
     unsigned long long fmask = 0ull;
     if (interior) {
-        // rows jw + d (d < 64) of this wave: element offsets from the start of
-        // row jw fit in 32 bits (the host limits n to below 2^24), and the
-        // row-dependent part of each offset is wave-uniform
+        // every pair valid, i > j: stored branch-free.  Rows jw + d (d < 64)
+        // of this wave: element offsets from the start of row jw fit in 32
+        // bits (the host limits n to below 2^24), and the row-dependent part
+        // of each offset is wave-uniform
         const int64_t jw = j0 + wr * 64;
         float *const outw =
             args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, jw) : jw * args.ldd) + i0;
@@ -406,29 +330,57 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
                 }
             }
     }
+    return fmask;
+}
 
-    // ---- exact FP32 recompute of the flagged pairs -------------------------
-    int total;
-    const int myoff = block_exclusive_scan<C::NW>(__popcll(fmask), scan_tmp, total);
-    if (total == 0) return;
-    // the epilogue's stores of flagged pairs land before the exact values
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    const int64_t ldx = args.ldx;
-    // Staging unit e = tid + THREADS q (q < FUNITS): tile row e >> 3 (A side
-    // rows first, then B side), features (e & 7) * 4..+3 of a 32-feature
-    // chunk.  The accumulators are dead here, so their registers hold a ring
-    // of FRING chunks in flight (the pass is bound by load latency, not
-    // bandwidth).
+// Tile position of a flag bit of this lane: (A row << 16) | B row.
+__device__ __forceinline__ unsigned flag_entry(int bit, int wr, int li0, int lane)
+{
+    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
+    const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    return ((unsigned)lj << 16) | (unsigned)(li0 + b * 32);
+}
+
+// Exact distance of a flagged pair (entry as flag_entry) of tile (i0, j0).
+template <int LAYOUT>
+__device__ __forceinline__ void store_exact(const GramArgs &args, int64_t i0, int64_t j0,
+                                            unsigned ent, float sum)
+{
+    const int64_t j = j0 + (ent >> 16), i = i0 + (ent & 0xffffu);
+    const float v = sqrtf(sum);
+    if (LAYOUT == OUT_PACKED) {
+        args.out[packed_base(args.n, j) + i] = v;
+    } else {
+        args.out[j * args.ldd + i] = v;
+        args.out[i * args.ldd + j] = v;
+    }
+}
+
+// Serial exact FP32 recompute of the tile's flagged pairs (every one of
+// them): the FP32 rows of both sides are re-streamed through LDS in
+// 32-feature chunks (two-level sums); the pairs go through an LDS list of
+// FCAP entries per pass.  myoff: this lane's rank among the flagged pairs;
+// total: their number.  The caller has let the epilogue's stores land.
+template <int LAYOUT, bool VEC4>
+__device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *smem, int64_t i0,
+                                           int64_t j0, unsigned long long fmask, int myoff,
+                                           int total, int wr, int li0, int lane)
+{
+    const int tid = threadIdx.x;
+    float *fS = reinterpret_cast<float *>(smem);
+    unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
+    const int64_t n = args.n, k = args.k, ldx = args.ldx;
+    // Staging unit e = tid + 1024 q (q < FUNITS): tile row e >> 3 (A side
+    // rows first, then B side), features (e & 7) * 4..+3 of a chunk; the
+    // unit's column is (tid & 7) * 4 for every q.  The accumulators are dead
+    // here, so their registers hold a ring of FRING chunks in flight (the
+    // pass is bound by load latency, not bandwidth).
     constexpr int FRING = 2;
-    constexpr int FUNITS = C::FUNITS, FPER = C::FPER;
-    // the unit's column is (tid & 7) * 4 for every q (THREADS is a multiple
-    // of 8); rows are kept as indices (fewer registers than pointers)
     int frow[FUNITS];
 #pragma unroll
     for (int q = 0; q < FUNITS; ++q) {
-        const int row = (tid + THREADS * q) >> 3;
-        int64_t g = row < BA ? j0 + row : i0 + (row - BA);
+        const int row = (tid + GTHREADS * q) >> 3;
+        const int64_t g = row < GBM ? j0 + row : i0 + (row - GBM);
         frow[q] = (int)(g < n ? g : n - 1);
     }
     const float *const fx = args.x + (tid & 7) * 4;
@@ -457,16 +409,16 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
     do {                                                                                   \
         __syncthreads();                                                                   \
         _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
-            const int e = tid + THREADS * q;                                               \
+            const int e = tid + GTHREADS * q;                                              \
             *reinterpret_cast<float4 *>(fS + (e >> 3) * FLD + (e & 7) * 4) = buf[q];       \
         }                                                                                  \
         __syncthreads();                                                                   \
         if ((chunk) + FRING < nf) GT_FLOAD(buf, (chunk) + FRING);                          \
         _Pragma("unroll") for (int c = 0; c < FPER; ++c) {                                 \
-            if (tid + THREADS * c < cnt) {                                                 \
+            if (tid + GTHREADS * c < cnt) {                                                \
                 const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD); \
                 const float4 *rb =                                                         \
-                    reinterpret_cast<const float4 *>(fS + (BA + (ent[c] & 0xffffu)) * FLD); \
+                    reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD); \
                 float part = 0.0f;                                                         \
                 _Pragma("unroll") for (int q = 0; q < FBK / 4; ++q) {                      \
                     const float4 u = ra[q], v = rb[q];                                     \
@@ -490,12 +442,7 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
             while (m) {
                 const int bit = __builtin_ctzll(m);
                 m &= m - 1;
-                if (rank >= base && rank < base + FCAP) {
-                    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
-                    const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    const int li = li0 + b * 32;
-                    list[rank - base] = ((unsigned)lj << 16) | (unsigned)li;
-                }
+                if (rank >= base && rank < base + FCAP) list[rank - base] = flag_entry(bit, wr, li0, lane);
                 ++rank;
             }
         }
@@ -509,7 +456,7 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
-            const int e = tid + THREADS * c;
+            const int e = tid + GTHREADS * c;
             ent[c] = e < cnt ? list[e] : 0u;
         }
         for (int ch = 0; ch < nf; ch += FRING) {
@@ -517,22 +464,85 @@ __global__ __launch_bounds__(WA * 256, 4) void gram_tile_kernel(
             if (ch + 1 < nf) GT_FSTEP(fr1, ch + 1);
         }
 #pragma unroll
-        for (int c = 0; c < FPER; ++c) {
-            if (tid + THREADS * c < cnt) {
-                const int64_t j = j0 + (ent[c] >> 16), i = i0 + (ent[c] & 0xffffu);
-                const float v = sqrtf(sum[c]);
-                if (LAYOUT == OUT_PACKED) {
-                    args.out[packed_base(n, j) + i] = v;
-                } else {
-                    args.out[j * args.ldd + i] = v;
-                    args.out[i * args.ldd + j] = v;
-                }
-            }
-        }
+        for (int c = 0; c < FPER; ++c)
+            if (tid + GTHREADS * c < cnt) store_exact<LAYOUT>(args, i0, j0, ent[c], sum[c]);
         __syncthreads();
     }
 #undef GT_FSTEP
 #undef GT_FLOAD
+}
+
+// One workgroup per 256 x 256 tile: main loop, epilogue, then the exact
+// pass of the flagged pairs.
+template <int SPLIT, int LAYOUT, bool VEC4>
+__global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
+{
+    constexpr int NT = Split<SPLIT>::NT;
+    typedef TileLds<NT> L;
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[L::BYTES];
+    float *nA = reinterpret_cast<float *>(smem + L::SIDE);
+    float *nB = nA + GBM, *iA = nB + GBM, *iB = iA + GBM;
+    int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 2, wc = wave & 3;
+
+    // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
+    // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
+    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int64_t st = id / (GSUPER * GSUPER);
+    const int w = (int)(id % (GSUPER * GSUPER));
+    int SI, SJ;
+    tri_tile(st, SI, SJ);
+    const int64_t I = (int64_t)SI * GSUPER + w / GSUPER;
+    const int64_t J = (int64_t)SJ * GSUPER + w % GSUPER;
+    if (I >= args.T || J > I) return;
+    const int64_t i0 = I * GBM, j0 = J * GBM;
+    load_side<SPLIT>(args, i0, j0, nA, nB, iA, iB);
+
+    const unsigned short *gsrc = chunk_source(args, i0, j0, wave, lane);
+    const int lrow = lane & 31;
+    const int roff = lrow * GROWB + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+    const int nchunks = (int)(args.kp / GBK);
+    // three stages: chunks c + 1 and c + 2 in flight while c is computed
+    issue_chunk<NT>(args, gsrc, smem, wave, 0);
+    if (nchunks > 1) issue_chunk<NT>(args, gsrc, smem + L::STAGE, wave, 1);
+    for (int c = 0; c < nchunks; ++c) {
+        // chunk c landed when at most the next chunk's NT loads per wave are
+        // outstanding (vector memory loads complete in order)
+        if (c + 1 < nchunks)
+            wait_vmcnt<NT>();
+        else
+            wait_vmcnt<0>();
+        // every wave's part of chunk c is in LDS, and every wave finished
+        // chunk c-1, whose stage the next issue overwrites.  A bare s_barrier:
+        // __syncthreads() would also drain the later chunks' loads.
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + 2 < nchunks) issue_chunk<NT>(args, gsrc, smem + ((c + 2) % 3) * L::STAGE, wave, c + 2);
+        mfma_chunk<SPLIT>(acc, smem + (c % 3) * L::STAGE + roff, wr, wc);
+    }
+    // the fix-up below reuses the stage memory
+    __syncthreads();
+
+    const bool interior = (I > J) && (i0 + GBM <= args.n);
+    const unsigned long long fmask =
+        gram_epilogue<SPLIT, LAYOUT>(acc, args, i0, j0, interior, wr, wc, lane, nA, nB, iA, iB);
+    int total;
+    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
+    if (total == 0) return;
+    // the epilogue's stores of flagged pairs land before the exact values
+    wait_vmcnt<0>();
+    __syncthreads();
+    gram_fixup<LAYOUT, VEC4>(args, smem, i0, j0, fmask, myoff, total, wr, wc * 64 + (lane & 31), lane);
 }
 
 // Column centre c_j of the Gram form.  Euclidean distances do not change
@@ -649,11 +659,6 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const float *__restrict__
     }
 }
 
-// The 256 x 256 tile (WA = 4).  Two workgroups per CU on 128 x 256 tiles
-// (WA = 2) produce the same results bit for bit but measured slower on
-// 65536 x 512: BF16 19.0 vs 16.0 ms on the mixture, FP16 14.0 vs 11.9 ms.
-constexpr int kGramWA = 4;
-
 template <int SPLIT, int LAYOUT, bool VEC4>
 hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
 {
@@ -661,8 +666,8 @@ hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
         args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
-        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4, kGramWA>),
-                           dim3((unsigned)args.ids_this), dim3(kGramWA * 256), 0, stream, args);
+        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4>), dim3((unsigned)args.ids_this),
+                           dim3(GTHREADS), 0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -800,7 +805,7 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
     args.T = (n + GBM - 1) / GBM;
     args.tau = tau;
     const int64_t S = (args.T + GSUPER - 1) / GSUPER;
-    const int64_t ids = S * (S + 1) / 2 * GSUPER * GSUPER * (4 / kGramWA);
+    const int64_t ids = S * (S + 1) / 2 * GSUPER * GSUPER;
     const bool vec4 = (k % 4 == 0) && (ldx % 4 == 0) &&
                       ((reinterpret_cast<uintptr_t>(x) & 15u) == 0);
     return fp16 ? launch_split<SPLIT_FP16>(layout, vec4, args, center, scales, ids, stream)
