@@ -300,6 +300,51 @@ __global__ __launch_bounds__(PTHREADS) void plain_tile_kernel(PlainArgs args)
         return;
     }
 
+    if (LAYOUT == OUT_PACKED) {
+        // Packed rows start at arbitrary offsets, so the values go out as
+        // whole runs of a row instead: 32 rows of the tile at a time are
+        // transposed through LDS (the staging array), then each wave writes
+        // rows with its 64 lanes on 64 consecutive i (two 256-byte runs per
+        // row).  The lane holding rows ty*4 + r (r < 4) and 64 + ty*4 + r - 4
+        // takes part in groups 0-1 and 2-3 respectively.
+        float *tr = &lds[0][0][0];   // 32 x TLD floats
+        constexpr int TLD = PBM + 1;
+        static_assert(32 * TLD * 4 <= (int)sizeof(lds), "transpose buffer");
+        const int wv = tid >> 6, ln = tid & 63;
+#pragma unroll
+        for (int grp = 0; grp < 4; ++grp) {
+            __syncthreads();
+            const int rbase = (grp & 1) * 32;
+            if (ty * 4 >= rbase && ty * 4 < rbase + 32) {
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int r = (grp < 2 ? 0 : 4) + r4;
+                    float *trow = tr + (ty * 4 + r4 - rbase) * TLD;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const float total = (OP == OP_MAXIMUM) ? part[r][c] : acc[r][c];
+                        trow[c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4)] =
+                            Term<OP>::finish(total, cnt[r][c] + clean_terms, args.inv_p, args.fk);
+                    }
+                }
+            }
+            __syncthreads();
+            const int64_t jg = j0 + grp * 32;
+#pragma unroll
+            for (int rr = 0; rr < PTHREADS / 64 * 2; ++rr) {
+                const int lr = wv + (PTHREADS / 64) * rr;
+                const int64_t j = jg + lr;
+                if (j >= args.nb) continue;
+                float *orow = args.out + packed_base(args.na, j);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t i = i0 + h * 64 + ln;
+                    if (i > j && i < args.na) orow[i] = tr[lr * TLD + h * 64 + ln];
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
