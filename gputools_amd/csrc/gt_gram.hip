@@ -41,21 +41,33 @@ constexpr int GBM = 256;        // tile side (points)
 constexpr int GBK = 16;         // features per main-loop chunk (one MFMA k-step)
 constexpr int KPAD = 32;        // the split terms are zero-padded to a multiple of this
 constexpr int GROWB = GBK * 2;  // bytes of one row of one term in one chunk (two 16 B halves)
-constexpr int GTHREADS = 1024;  // 16 waves in a 4 (A) x 4 (B) grid, each 64 x 64 pairs
 constexpr int GSUPER = 4;       // super-tile side in tiles (L2 locality)
-constexpr int FBK = 32;         // features per serial fix-up chunk
-constexpr int FLD = FBK + 4;    // serial fix-up LDS row length (floats, 16 B aligned rows)
-constexpr int FCAP = 1024;      // flagged pairs per serial fix-up pass
+constexpr int FBK = 32;         // features per fix-up chunk
+constexpr int FLD = FBK + 4;    // fix-up LDS row length (floats, 16 B aligned rows)
+constexpr int FCAP = 1024;      // flagged pairs per fix-up pass
 // Gram-form values are finished with the hardware square root (v_sqrt_f32)
 // plus one residual correction (sqrt_rn), which rounds correctly for normal
 // inputs; d^2 below 2^-100 goes to the exact pass instead.
 constexpr float GRAM_SQRT_MIN = 0x1p-100f;
-constexpr int FPER = FCAP / GTHREADS;
-constexpr int FUNITS = 2 * GBM * FBK / 4 / GTHREADS;  // float4 staging units per thread
 constexpr int TERM_BYTES = GBM * GROWB;        // one split term of one side, one chunk (8 KB)
-constexpr int FIX_BYTES = 2 * GBM * FLD * 4;   // serial fix-up staging, both sides
+constexpr int FIX_BYTES = 2 * GBM * FLD * 4;   // fix-up staging, both sides
 constexpr int LIST_BYTES = FCAP * 4;
 constexpr int SIDE_BYTES = 4 * GBM * 4 + 16 * 4;  // norms A, B, inverse scales A, B, scan
+
+// Wave layout of a 256 x 256 tile.  NW waves in a 4 (A rows) x NW/4 (B rows)
+// grid; each wave holds 64 x (256 / (NW/4)) pairs as 2 x NB blocks of 32 x 32
+// accumulators (NB = 8 / (NW/4)).  NW = 16: 64 x 64 per wave, 1024 threads,
+// 128 VGPRs per lane.  NW = 8: 64 x 128 per wave, 512 threads, up to 256
+// VGPRs per lane (the accumulators take 128).
+template <int NW>
+struct Waves {
+    static constexpr int THREADS = NW * 64;
+    static constexpr int WC = NW / 4;             // B-row groups
+    static constexpr int NB = 8 / WC;             // 32-row accumulator blocks along B per wave
+    static constexpr int FUNITS = 2 * GBM * FBK / 4 / THREADS;  // fix-up float4 units per thread
+    static constexpr int FPER = FCAP / THREADS;   // fix-up list entries per thread
+    static constexpr int GPW = 16 / NW;           // 32-row load groups per wave per term
+};
 
 // LDS: [split terms, three stages | (aliased) fix-up staging + pair list]
 // [side data]
@@ -119,7 +131,8 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     }
 }
 
-// Exclusive scan of one int per thread over the 1024-thread workgroup.
+// Exclusive scan of one int per thread over a workgroup of NW waves.
+template <int NW>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -134,7 +147,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
     int before = 0;
     total = 0;
 #pragma unroll
-    for (int w = 0; w < GTHREADS / 64; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const int c = tmp[w];
         if (w < wave) before += c;
         total += c;
@@ -158,36 +171,48 @@ typedef const __attribute__((address_space(1))) void gbl_void_t;
 // when swz(R) = ((R >> 2) ^ (R >> 3)) & 1, so that the b128 operand reads of
 // 8 or 16 consecutive rows hit distinct banks without padding.
 // global_load_lds writes lane l's 16 B at the wave's LDS base + 16 l, so each
-// wave instruction fills 32 rows of one term: wave w fills rows
-// 32 (w % 8) .. + 31 of side w / 8.  Rows past n are clamped to row n - 1:
-// they only feed pairs that are never stored.
-__device__ __forceinline__ const unsigned short *chunk_source(const GramArgs &args, int64_t i0,
-                                                              int64_t j0, int wave, int lane)
+// wave instruction fills 32 rows of one term.  The 16 groups of 32 rows
+// (8 per side) go to the waves round robin: wave w fills groups w, w + NW,
+// ... (group g: side g / 8, rows 32 (g % 8) ..).  Rows past n are clamped to
+// row n - 1: they only feed pairs that are never stored.
+template <int NW>
+__device__ __forceinline__ void chunk_sources(const GramArgs &args, int64_t i0, int64_t j0,
+                                              int wave, int lane,
+                                              const unsigned short *(&gsrc)[Waves<NW>::GPW])
 {
-    const int row = (wave & 7) * 32 + (lane >> 1);
-    const int half = (lane & 1) ^ (((row >> 2) ^ (row >> 3)) & 1);
-    const int64_t r = ((wave >> 3) ? i0 : j0) + row;
-    return args.terms + (r < args.n ? r : args.n - 1) * args.kp + half * 8;
+    const int rr = lane >> 1;
+    const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
+#pragma unroll
+    for (int q = 0; q < Waves<NW>::GPW; ++q) {
+        const int g = wave + q * NW;
+        const int64_t r = ((g >> 3) ? i0 : j0) + (g & 7) * 32 + rr;
+        gsrc[q] = args.terms + (r < args.n ? r : args.n - 1) * args.kp + half * 8;
+    }
 }
 
-template <int NT>
-__device__ __forceinline__ void issue_chunk(const GramArgs &args, const unsigned short *gsrc,
+template <int NT, int NW>
+__device__ __forceinline__ void issue_chunk(const GramArgs &args,
+                                            const unsigned short *const (&gsrc)[Waves<NW>::GPW],
                                             unsigned char *stage, int wave, int chunk)
 {
     const int64_t term_stride = args.n * args.kp;
     const int64_t kb = (int64_t)chunk * GBK;
-    unsigned char *dst = stage + (wave >> 3) * NT * TERM_BYTES + (wave & 7) * 1024;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-        __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc + t * term_stride + kb),
-                                         (lds_void_t *)(dst + t * TERM_BYTES), 16, 0, 0);
+    for (int q = 0; q < Waves<NW>::GPW; ++q) {
+        const int g = wave + q * NW;
+        unsigned char *dst = stage + (g >> 3) * NT * TERM_BYTES + (g & 7) * 1024;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[q] + t * term_stride + kb),
+                                             (lds_void_t *)(dst + t * TERM_BYTES), 16, 0, 0);
+    }
 }
 
-// One chunk of MFMA products for this wave's 64 x 64 pairs (acc[a][b]: A
-// rows wr*64 + a*32.., B rows wc*64 + b*32..); sg = stage + the lane's
-// operand offset.
-template <int SPLIT>
-__device__ __forceinline__ void mfma_chunk(f32x16_t (&acc)[2][2], const unsigned char *sg, int wr,
+// One chunk of MFMA products for this wave's pairs (acc[a][b]: A rows
+// wr*64 + a*32.., B rows wc*NB*32 + b*32..); sg = stage + the lane's operand
+// offset.
+template <int SPLIT, int NB>
+__device__ __forceinline__ void mfma_chunk(f32x16_t (&acc)[2][NB], const unsigned char *sg, int wr,
                                            int wc)
 {
     typedef typename Split<SPLIT>::vec vec;
@@ -199,12 +224,12 @@ __device__ __forceinline__ void mfma_chunk(f32x16_t (&acc)[2][2], const unsigned
         for (int a = 0; a < 2; ++a)
             av[t][a] = *reinterpret_cast<const vec *>(sg + t * TERM_BYTES + (wr * 64 + a * 32) * GROWB);
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NB; ++b) {
         vec bv[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
             bv[t] = *reinterpret_cast<const vec *>(sg + (NT + t) * TERM_BYTES +
-                                                   (wc * 64 + b * 32) * GROWB);
+                                                   (wc * NB * 32 + b * 32) * GROWB);
         // hh' first, then the cross terms in decreasing magnitude
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -225,43 +250,44 @@ template <int SPLIT>
 __device__ __forceinline__ void load_side(const GramArgs &args, int64_t i0, int64_t j0, float *nA,
                                           float *nB, float *iA, float *iB)
 {
-    const int tid = threadIdx.x;
-    if (tid < GBM) {
-        const int64_t r = j0 + tid;
-        nA[tid] = r < args.n ? args.norms[r] : 0.0f;
-        if (SPLIT == SPLIT_FP16) iA[tid] = r < args.n ? args.iscale[r] : 1.0f;
-    } else if (tid < 2 * GBM) {
-        const int64_t r = i0 + (tid - GBM);
-        nB[tid - GBM] = r < args.n ? args.norms[r] : 0.0f;
-        if (SPLIT == SPLIT_FP16) iB[tid - GBM] = r < args.n ? args.iscale[r] : 1.0f;
+    for (int t = threadIdx.x; t < 2 * GBM; t += blockDim.x) {
+        if (t < GBM) {
+            const int64_t r = j0 + t;
+            nA[t] = r < args.n ? args.norms[r] : 0.0f;
+            if (SPLIT == SPLIT_FP16) iA[t] = r < args.n ? args.iscale[r] : 1.0f;
+        } else {
+            const int64_t r = i0 + (t - GBM);
+            nB[t - GBM] = r < args.n ? args.norms[r] : 0.0f;
+            if (SPLIT == SPLIT_FP16) iB[t - GBM] = r < args.n ? args.iscale[r] : 1.0f;
+        }
     }
 }
 
 // Epilogue of one tile: d^2 = ||x||^2 + ||y||^2 - 2g for this wave's pairs;
 // every valid pair that is not flagged (d^2 >= tau (||x||^2 + ||y||^2)) is
-// stored.  Returns the lane's flag bits: bit (2a + b) * 16 + r marks
-// accumulator element r of block (a, b).  32x32 accumulator layout: element
-// r of lane l is row 8(r/4) + 4(l/32) + r%4 (A side, j) and column l%32 (B
+// stored.  Flag bits: index (a * NB + b) * 16 + r marks accumulator element
+// r of block (a, b), in fm[index / 64].  32x32 accumulator layout: element r
+// of lane l is row 8(r/4) + 4(l/32) + r%4 (A side, j) and column l%32 (B
 // side, i); lanes 0..31 hold 32 consecutive i of one j, so packed stores are
 // contiguous runs.
-template <int SPLIT, int LAYOUT>
-__device__ __forceinline__ unsigned long long gram_epilogue(const f32x16_t (&acc)[2][2],
-                                                            const GramArgs &args, int64_t i0,
-                                                            int64_t j0, bool interior, int wr,
-                                                            int wc, int lane, const float *nA,
-                                                            const float *nB, const float *iA,
-                                                            const float *iB)
+template <int SPLIT, int LAYOUT, int NB>
+__device__ __forceinline__ void gram_epilogue(const f32x16_t (&acc)[2][NB], const GramArgs &args,
+                                              int64_t i0, int64_t j0, bool interior, int wr,
+                                              int wc, int lane, const float *nA, const float *nB,
+                                              const float *iA, const float *iB,
+                                              unsigned long long (&fm)[NB / 2])
 {
     const int64_t n = args.n;
-    const int li0 = wc * 64 + (lane & 31);
-    float ni[2], sb[2];
+    const int li0 = wc * NB * 32 + (lane & 31);
+    float ni[NB], sb[NB];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NB; ++b) {
         ni[b] = nB[li0 + b * 32];
         sb[b] = (SPLIT == SPLIT_FP16) ? iB[li0 + b * 32] : 1.0f;
     }
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) fm[q] = 0ull;
     const float tau = args.tau;
-    unsigned long long fmask = 0ull;
     if (interior) {
         // every pair valid, i > j: stored branch-free.  Rows jw + d (d < 64)
         // of this wave: element offsets from the start of row jw fit in 32
@@ -284,13 +310,14 @@ __device__ __forceinline__ unsigned long long gram_epilogue(const f32x16_t (&acc
                 const float nj = nA[lj];
                 const float sj = (SPLIT == SPLIT_FP16) ? iA[lj] : 1.0f;
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < NB; ++b) {
                     float g = acc[a][b][r];
                     if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
                     const float ss = nj + ni[b];
                     const float d2 = fmaf(-2.0f, g, ss);
                     const bool ok = d2 >= fmaf(tau, ss, GRAM_SQRT_MIN);
-                    fmask |= ok ? 0ull : (1ull << ((a * 2 + b) * 16 + r));
+                    const int bit = (a * NB + b) * 16 + r;
+                    fm[bit >> 6] |= ok ? 0ull : (1ull << (bit & 63));
                     const float v = sqrt_rn(d2);
                     outw[ro + (unsigned)(b * 32)] = v;
                     if (LAYOUT == OUT_FULL) args.out[(i0 + li0 + b * 32) * args.ldd + jw + lj - wr * 64] = v;
@@ -308,7 +335,7 @@ __device__ __forceinline__ unsigned long long gram_epilogue(const f32x16_t (&acc
                 float *rowp = args.out + ((LAYOUT == OUT_PACKED) ? packed_base(n, j) : j * args.ldd) +
                               i0 + li0;
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < NB; ++b) {
                     float g = acc[a][b][r];
                     // undo the row scales one at a time (their product can overflow)
                     if (SPLIT == SPLIT_FP16) g = (g * sj) * sb[b];
@@ -323,20 +350,21 @@ __device__ __forceinline__ unsigned long long gram_epilogue(const f32x16_t (&acc
                             rowp[b * 32] = v;
                             if (LAYOUT == OUT_FULL) args.out[(i0 + li) * args.ldd + j] = v;
                         } else {
-                            fmask |= 1ull << ((a * 2 + b) * 16 + r);
+                            const int bit = (a * NB + b) * 16 + r;
+                            fm[bit >> 6] |= 1ull << (bit & 63);
                         }
                     }
                     if (LAYOUT == OUT_FULL && i0 + li == j && j < n) rowp[b * 32] = 0.0f;
                 }
             }
     }
-    return fmask;
 }
 
-// Tile position of a flag bit of this lane: (A row << 16) | B row.
+// Tile position of flag index `bit` of this lane: (A row << 16) | B row.
+template <int NB>
 __device__ __forceinline__ unsigned flag_entry(int bit, int wr, int li0, int lane)
 {
-    const int a = bit >> 5, b = (bit >> 4) & 1, r = bit & 15;
+    const int a = bit / (NB * 16), b = (bit / 16) % NB, r = bit & 15;
     const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     return ((unsigned)lj << 16) | (unsigned)(li0 + b * 32);
 }
@@ -356,21 +384,24 @@ __device__ __forceinline__ void store_exact(const GramArgs &args, int64_t i0, in
     }
 }
 
-// Serial exact FP32 recompute of the tile's flagged pairs (every one of
-// them): the FP32 rows of both sides are re-streamed through LDS in
-// 32-feature chunks (two-level sums); the pairs go through an LDS list of
-// FCAP entries per pass.  myoff: this lane's rank among the flagged pairs;
-// total: their number.  The caller has let the epilogue's stores land.
-template <int LAYOUT, bool VEC4>
+// Exact FP32 recompute of the tile's flagged pairs: the FP32 rows of both
+// sides are re-streamed through LDS in 32-feature chunks (two-level sums);
+// the pairs go through an LDS list of FCAP entries per pass.  myoff: this
+// lane's rank among the flagged pairs; total: their number.  The caller has
+// let the epilogue's stores land.
+template <int LAYOUT, bool VEC4, int NW>
 __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *smem, int64_t i0,
-                                           int64_t j0, unsigned long long fmask, int myoff,
-                                           int total, int wr, int li0, int lane)
+                                           int64_t j0,
+                                           const unsigned long long (&fm)[Waves<NW>::NB / 2],
+                                           int myoff, int total, int wr, int li0, int lane)
 {
+    typedef Waves<NW> W;
+    constexpr int THREADS = W::THREADS, FUNITS = W::FUNITS, FPER = W::FPER, NB = W::NB;
     const int tid = threadIdx.x;
     float *fS = reinterpret_cast<float *>(smem);
     unsigned int *list = reinterpret_cast<unsigned int *>(smem + FIX_BYTES);
     const int64_t n = args.n, k = args.k, ldx = args.ldx;
-    // Staging unit e = tid + 1024 q (q < FUNITS): tile row e >> 3 (A side
+    // Staging unit e = tid + THREADS q (q < FUNITS): tile row e >> 3 (A side
     // rows first, then B side), features (e & 7) * 4..+3 of a chunk; the
     // unit's column is (tid & 7) * 4 for every q.  The accumulators are dead
     // here, so their registers hold a ring of FRING chunks in flight (the
@@ -379,7 +410,7 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
     int frow[FUNITS];
 #pragma unroll
     for (int q = 0; q < FUNITS; ++q) {
-        const int row = (tid + GTHREADS * q) >> 3;
+        const int row = (tid + THREADS * q) >> 3;
         const int64_t g = row < GBM ? j0 + row : i0 + (row - GBM);
         frow[q] = (int)(g < n ? g : n - 1);
     }
@@ -409,13 +440,13 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
     do {                                                                                   \
         __syncthreads();                                                                   \
         _Pragma("unroll") for (int q = 0; q < FUNITS; ++q) {                               \
-            const int e = tid + GTHREADS * q;                                              \
+            const int e = tid + THREADS * q;                                               \
             *reinterpret_cast<float4 *>(fS + (e >> 3) * FLD + (e & 7) * 4) = buf[q];       \
         }                                                                                  \
         __syncthreads();                                                                   \
         if ((chunk) + FRING < nf) GT_FLOAD(buf, (chunk) + FRING);                          \
         _Pragma("unroll") for (int c = 0; c < FPER; ++c) {                                 \
-            if (tid + GTHREADS * c < cnt) {                                                \
+            if (tid + THREADS * c < cnt) {                                                 \
                 const float4 *ra = reinterpret_cast<const float4 *>(fS + (ent[c] >> 16) * FLD); \
                 const float4 *rb =                                                         \
                     reinterpret_cast<const float4 *>(fS + (GBM + (ent[c] & 0xffffu)) * FLD); \
@@ -438,12 +469,16 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
     for (int base = 0; base < total; base += FCAP) {
         {
             int rank = myoff;
-            unsigned long long m = fmask;
-            while (m) {
-                const int bit = __builtin_ctzll(m);
-                m &= m - 1;
-                if (rank >= base && rank < base + FCAP) list[rank - base] = flag_entry(bit, wr, li0, lane);
-                ++rank;
+#pragma unroll
+            for (int qw = 0; qw < NB / 2; ++qw) {
+                unsigned long long m = fm[qw];
+                while (m) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (rank >= base && rank < base + FCAP)
+                        list[rank - base] = flag_entry<NB>(qw * 64 + bit, wr, li0, lane);
+                    ++rank;
+                }
             }
         }
         const int cnt = (total - base) < FCAP ? (total - base) : FCAP;
@@ -456,7 +491,7 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < FPER; ++c) {
-            const int e = tid + GTHREADS * c;
+            const int e = tid + THREADS * c;
             ent[c] = e < cnt ? list[e] : 0u;
         }
         for (int ch = 0; ch < nf; ch += FRING) {
@@ -465,7 +500,7 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
         }
 #pragma unroll
         for (int c = 0; c < FPER; ++c)
-            if (tid + GTHREADS * c < cnt) store_exact<LAYOUT>(args, i0, j0, ent[c], sum[c]);
+            if (tid + THREADS * c < cnt) store_exact<LAYOUT>(args, i0, j0, ent[c], sum[c]);
         __syncthreads();
     }
 #undef GT_FSTEP
@@ -474,10 +509,11 @@ __device__ __forceinline__ void gram_fixup(const GramArgs &args, unsigned char *
 
 // One workgroup per 256 x 256 tile: main loop, epilogue, then the exact
 // pass of the flagged pairs.
-template <int SPLIT, int LAYOUT, bool VEC4>
-__global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
+template <int SPLIT, int LAYOUT, bool VEC4, int NW>
+__global__ __launch_bounds__(NW * 64) void gram_tile_kernel(GramArgs args)
 {
-    constexpr int NT = Split<SPLIT>::NT;
+    typedef Waves<NW> W;
+    constexpr int NT = Split<SPLIT>::NT, NB = W::NB, GPW = W::GPW;
     typedef TileLds<NT> L;
     __shared__ __attribute__((aligned(1024))) unsigned char smem[L::BYTES];
     float *nA = reinterpret_cast<float *>(smem + L::SIDE);
@@ -485,7 +521,7 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     int *scan_tmp = reinterpret_cast<int *>(iB + GBM);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave >> 2, wc = wave & 3;
+    const int wr = wave / W::WC, wc = wave % W::WC;
 
     // Super-tiled lower-triangular schedule: super-tile s covers GSUPER x
     // GSUPER tiles; tiles above the diagonal or past the edge exit at once.
@@ -500,26 +536,27 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
     const int64_t i0 = I * GBM, j0 = J * GBM;
     load_side<SPLIT>(args, i0, j0, nA, nB, iA, iB);
 
-    const unsigned short *gsrc = chunk_source(args, i0, j0, wave, lane);
+    const unsigned short *gsrc[GPW];
+    chunk_sources<NW>(args, i0, j0, wave, lane, gsrc);
     const int lrow = lane & 31;
     const int roff = lrow * GROWB + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
-    f32x16_t acc[2][2];
+    f32x16_t acc[2][NB];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
     const int nchunks = (int)(args.kp / GBK);
     // three stages: chunks c + 1 and c + 2 in flight while c is computed
-    issue_chunk<NT>(args, gsrc, smem, wave, 0);
-    if (nchunks > 1) issue_chunk<NT>(args, gsrc, smem + L::STAGE, wave, 1);
+    issue_chunk<NT, NW>(args, gsrc, smem, wave, 0);
+    if (nchunks > 1) issue_chunk<NT, NW>(args, gsrc, smem + L::STAGE, wave, 1);
     for (int c = 0; c < nchunks; ++c) {
-        // chunk c landed when at most the next chunk's NT loads per wave are
-        // outstanding (vector memory loads complete in order)
+        // chunk c landed when at most the next chunk's loads of this wave
+        // are outstanding (vector memory loads complete in order)
         if (c + 1 < nchunks)
-            wait_vmcnt<NT>();
+            wait_vmcnt<NT * GPW>();
         else
             wait_vmcnt<0>();
         // every wave's part of chunk c is in LDS, and every wave finished
@@ -527,22 +564,27 @@ __global__ __launch_bounds__(GTHREADS) void gram_tile_kernel(GramArgs args)
         // __syncthreads() would also drain the later chunks' loads.
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (c + 2 < nchunks) issue_chunk<NT>(args, gsrc, smem + ((c + 2) % 3) * L::STAGE, wave, c + 2);
-        mfma_chunk<SPLIT>(acc, smem + (c % 3) * L::STAGE + roff, wr, wc);
+        if (c + 2 < nchunks)
+            issue_chunk<NT, NW>(args, gsrc, smem + ((c + 2) % 3) * L::STAGE, wave, c + 2);
+        mfma_chunk<SPLIT, NB>(acc, smem + (c % 3) * L::STAGE + roff, wr, wc);
     }
     // the fix-up below reuses the stage memory
     __syncthreads();
 
     const bool interior = (I > J) && (i0 + GBM <= args.n);
-    const unsigned long long fmask =
-        gram_epilogue<SPLIT, LAYOUT>(acc, args, i0, j0, interior, wr, wc, lane, nA, nB, iA, iB);
+    unsigned long long fm[NB / 2];
+    gram_epilogue<SPLIT, LAYOUT, NB>(acc, args, i0, j0, interior, wr, wc, lane, nA, nB, iA, iB, fm);
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) cnt += __popcll(fm[q]);
     int total;
-    const int myoff = block_exclusive_scan(__popcll(fmask), scan_tmp, total);
+    const int myoff = block_exclusive_scan<NW>(cnt, scan_tmp, total);
     if (total == 0) return;
     // the epilogue's stores of flagged pairs land before the exact values
     wait_vmcnt<0>();
     __syncthreads();
-    gram_fixup<LAYOUT, VEC4>(args, smem, i0, j0, fmask, myoff, total, wr, wc * 64 + (lane & 31), lane);
+    gram_fixup<LAYOUT, VEC4, NW>(args, smem, i0, j0, fm, myoff, total, wr,
+                                 wc * NB * 32 + (lane & 31), lane);
 }
 
 // Column centre c_j of the Gram form.  Euclidean distances do not change
@@ -659,6 +701,12 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const float *__restrict__
     }
 }
 
+// 16 waves (64 x 64 pairs each).  8 waves of 64 x 128 pairs (512 threads,
+// about 200 VGPRs) give the same results bit for bit and measured slower:
+// 16.8 vs 15.9 ms on the 65536 x 512 mixture, 12.8 vs 12.5 ms on N(0, 1)
+// (BF16); the exact pass has half the loads in flight.
+constexpr int kGramWaves = 16;
+
 template <int SPLIT, int LAYOUT, bool VEC4>
 hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
 {
@@ -666,8 +714,8 @@ hipError_t launch_tiles(GramArgs &args, int64_t ids, hipStream_t stream)
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
         args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
-        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4>), dim3((unsigned)args.ids_this),
-                           dim3(GTHREADS), 0, stream, args);
+        hipLaunchKernelGGL((gram_tile_kernel<SPLIT, LAYOUT, VEC4, kGramWaves>),
+                           dim3((unsigned)args.ids_this), dim3(kGramWaves * 64), 0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
