@@ -7,10 +7,12 @@
 // tile kernel counts popcount(a & b) per pair and word (one AND and one
 // v_bcnt_u32_b32, which adds to an accumulator).  From the row counts pa, pb:
 //   on in exactly one row = pa + pb - 2 (a & b),  on in either = pa + pb - (a & b).
-// The counts are exact integers; the ratio uses the correctly rounded FP32
-// divide, which equals the FP32 rounding of the FP64 ratio R computes
-// (double rounding of a quotient from binary64 to binary32 is innocuous), so
-// the result is bit-exact against stats::dist rounded to FP32.
+// The counts are exact integers; their ratio is rounded correctly to FP32,
+// which equals the FP32 rounding of the FP64 ratio R computes (double
+// rounding of a quotient from binary64 to binary32 is innocuous), so the
+// result is bit-exact against stats::dist rounded to FP32.  Up to
+// SMALL_DIV_MAX features the quotient takes a reciprocal and two fused
+// corrections instead of the IEEE divide sequence (small_quotient).
 #include "gt_common.h"
 #include "gt_internal.h"
 
@@ -22,6 +24,32 @@ constexpr int BBK = 16;        // words per LDS chunk (512 features)
 constexpr int BTHREADS = 256;
 constexpr int BLD = BBM + 4;
 constexpr int BLOADS = BBM * BBK / BTHREADS;
+
+// Largest count for which small_quotient is exact (checked exhaustively).
+constexpr int64_t SMALL_DIV_MAX = 131072;
+
+// a / b rounded correctly, for integers 0 <= a <= b <= SMALL_DIV_MAX held
+// exactly as floats: y = v_rcp_f32(b) (within 1 ulp of 1/b), q0 = a y, the
+// remainder a - q0 b is exact in one fma, and q0 + r y rounds to the
+// correctly rounded quotient.  tests/exhaustive/small_div.c checks every
+// such pair against each float within one ulp of 1/b.
+__device__ __forceinline__ float small_quotient(float a, float b)
+{
+    const float y = __builtin_amdgcn_rcpf(b);
+    const float q0 = __fmul_rn(a, y);
+    const float r = fmaf(-q0, b, a);
+    return fmaf(r, y, q0);
+}
+
+// R's binary distance from the counts: no bit on in either row -> 0, else
+// (#on in exactly one) / (#on in either).
+template <bool FASTDIV>
+__device__ __forceinline__ float binary_value(uint32_t pa_plus_pb, uint32_t both)
+{
+    const uint32_t nx = pa_plus_pb - 2u * both, no = pa_plus_pb - both;
+    const float q = FASTDIV ? small_quotient((float)nx, (float)no) : (float)nx / (float)no;
+    return no == 0 ? 0.0f : q;
+}
 
 __global__ __launch_bounds__(256) void binary_pack_kernel(const float *__restrict__ x, int64_t n,
                                                           int64_t k, int64_t ldx,
@@ -59,12 +87,18 @@ struct BinaryArgs {
     bool vec;           // ldd and out allow 16-byte stores (pitched layouts)
 };
 
-template <int LAYOUT>
+// One 128 x 128 tile of pairs per workgroup.  Wave w holds tile rows (B
+// side, j) w * 32 .. w * 32 + 31 and lane l columns (A side, i) l and l + 64:
+// 32 x 2 counts per lane.  A row's words are the same for the whole wave
+// (broadcast LDS reads), and a store instruction writes 64 consecutive
+// values of one output row, from a wave-uniform row base.
+template <int LAYOUT, bool FASTDIV>
 __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
 {
-    __shared__ uint32_t lds[2][BBK][BLD];
+    __shared__ __attribute__((aligned(16))) uint32_t lds[2][BBK][BLD];
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t t = args.tile_offset + xcd_remap(blockIdx.x, args.tiles_this);
     int I, J;
     if (LAYOUT == OUT_CROSS) {
@@ -74,12 +108,11 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
         tri_tile(t, I, J);
     }
     const int64_t i0 = (int64_t)I * BBM, j0 = (int64_t)J * BBM;
+    const int rw = wave * 32;   // first tile row of this wave
 
-    uint32_t nand[8][8];
+    uint32_t nand[32][2];
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) nand[r][c] = 0;
+    for (int r = 0; r < 32; ++r) nand[r][0] = nand[r][1] = 0;
 
     for (int64_t wb = 0; wb < args.words; wb += BBK) {
         __syncthreads();
@@ -94,79 +127,87 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
         }
         __syncthreads();
         const int nw = (args.words - wb) < BBK ? (int)(args.words - wb) : BBK;
-#pragma unroll 2
         for (int wc = 0; wc < nw; ++wc) {
-            const uint4 c0 = *reinterpret_cast<const uint4 *>(&lds[0][wc][tx * 4]);
-            const uint4 c1 = *reinterpret_cast<const uint4 *>(&lds[0][wc][64 + tx * 4]);
-            const uint4 r0 = *reinterpret_cast<const uint4 *>(&lds[1][wc][ty * 4]);
-            const uint4 r1 = *reinterpret_cast<const uint4 *>(&lds[1][wc][64 + ty * 4]);
-            const uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-            const uint32_t rv[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+            const uint32_t c0 = lds[0][wc][lane], c1 = lds[0][wc][64 + lane];
 #pragma unroll
-            for (int r = 0; r < 8; ++r)
+            for (int q = 0; q < 8; ++q) {
+                const uint4 rq = *reinterpret_cast<const uint4 *>(&lds[1][wc][rw + 4 * q]);
+                const uint32_t rv[4] = {rq.x, rq.y, rq.z, rq.w};
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    uint32_t s = nand[r][c];
-                    s += (uint32_t)__popc(cv[c] & rv[r]);
-                    asm("" : "+v"(s));   // keeps v_bcnt_u32_b32 accumulating
-                    nand[r][c] = s;
+                for (int u = 0; u < 4; ++u) {
+                    uint32_t s0 = nand[4 * q + u][0], s1 = nand[4 * q + u][1];
+                    s0 += (uint32_t)__popc(c0 & rv[u]);
+                    s1 += (uint32_t)__popc(c1 & rv[u]);
+                    asm("" : "+v"(s0), "+v"(s1));   // keeps v_bcnt_u32_b32 accumulating
+                    nand[4 * q + u][0] = s0;
+                    nand[4 * q + u][1] = s1;
                 }
+            }
         }
     }
-    uint32_t pc[8], pr[8];   // set bits of the lane's rows on each side
+    // set bits of the lane's two columns; the rows' counts are wave-uniform
+    uint32_t pc[2];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int64_t i = i0 + (u < 4 ? tx * 4 + u : 64 + tx * 4 + (u - 4));
-        const int64_t j = j0 + (u < 4 ? ty * 4 + u : 64 + ty * 4 + (u - 4));
-        pc[u] = i < args.na ? args.a[i * args.ldw + args.words] : 0u;
-        pr[u] = j < args.nb ? args.b[j * args.ldw + args.words] : 0u;
+    for (int c = 0; c < 2; ++c) {
+        const int64_t i = i0 + lane + 64 * c;
+        pc[c] = i < args.na ? args.a[i * args.ldw + args.words] : 0u;
     }
+    const uint32_t *const prow = args.b + args.words;
+    const bool whole = i0 + BBM <= args.na && j0 + BBM <= args.nb;
 
-    // Whole tiles of the pitched layouts (cross, and the off-diagonal tiles
-    // of the symmetric one) store 16 bytes at a time in both directions.
-    if (LAYOUT != OUT_PACKED && args.vec && i0 + BBM <= args.na && j0 + BBM <= args.nb &&
-        (LAYOUT == OUT_CROSS || I > J)) {
-        float v[8][8];
+    if (LAYOUT == OUT_PACKED && I > J && whole) {
+        // every pair valid and i > j
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
+        for (int r = 0; r < 32; ++r) {
+            const int64_t j = j0 + rw + r;
+            const uint32_t pr = prow[j * args.ldw];
+            float *rowp = args.out + packed_base(args.na, j) + i0;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const uint32_t both = pc[c] + pr[r];
-                const uint32_t nx = both - 2u * nand[r][c], no = both - nand[r][c];
-                v[r][c] = (no == 0) ? 0.0f : (float)nx / (float)no;
+            for (int c = 0; c < 2; ++c) rowp[lane + 64 * c] = binary_value<FASTDIV>(pc[c] + pr, nand[r][c]);
+        }
+        return;
+    }
+    if (LAYOUT != OUT_PACKED && whole && (LAYOUT == OUT_CROSS || I > J)) {
+        // direct rows: 64 consecutive values per store instruction
+        float v[32][2];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int64_t j = j0 + rw + r;
+            const uint32_t pr = prow[j * args.ldw];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                v[r][c] = binary_value<FASTDIV>(pc[c] + pr, nand[r][c]);
+                args.out[j * args.ldd + i0 + lane + 64 * c] = v[r][c];
             }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-                *reinterpret_cast<float4 *>(args.out + j * args.ldd + i0 + h * 64 + tx * 4) =
-                    make_float4(v[r][4 * h], v[r][4 * h + 1], v[r][4 * h + 2], v[r][4 * h + 3]);
         }
         if (LAYOUT == OUT_FULL) {
+            // mirrored: a lane's 32 rows are 128 consecutive bytes of row i
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+            for (int c = 0; c < 2; ++c) {
+                float *dst = args.out + (i0 + lane + 64 * c) * args.ldd + j0 + rw;
+                if (args.vec) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    *reinterpret_cast<float4 *>(args.out + i * args.ldd + j0 + h * 64 + ty * 4) =
-                        make_float4(v[4 * h][c], v[4 * h + 1][c], v[4 * h + 2][c], v[4 * h + 3][c]);
+                    for (int q = 0; q < 8; ++q)
+                        *reinterpret_cast<float4 *>(dst + 4 * q) =
+                            make_float4(v[4 * q][c], v[4 * q + 1][c], v[4 * q + 2][c], v[4 * q + 3][c]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) dst[r] = v[r][c];
+                }
             }
         }
         return;
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int64_t j = j0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
-        if (j >= args.nb) continue;
+    for (int r = 0; r < 32; ++r) {
+        const int64_t j = j0 + rw + r;
+        if (j >= args.nb) break;
+        const uint32_t pr = prow[j * args.ldw];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int64_t i = i0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+        for (int c = 0; c < 2; ++c) {
+            const int64_t i = i0 + lane + 64 * c;
             if (i >= args.na) continue;
-            // R: no bit on in either row -> 0; else (#on in one)/(#on in either)
-            const uint32_t both = pc[c] + pr[r];
-            const uint32_t nx = both - 2u * nand[r][c], no = both - nand[r][c];
-            const float v = (no == 0) ? 0.0f : (float)nx / (float)no;
+            const float v = binary_value<FASTDIV>(pc[c] + pr, nand[r][c]);
             if (LAYOUT == OUT_PACKED) {
                 if (i > j) args.out[packed_base(args.na, j) + i] = v;
             } else if (LAYOUT == OUT_FULL) {
@@ -181,6 +222,17 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
             }
         }
     }
+}
+
+template <bool FASTDIV>
+void launch_layout(int layout, dim3 grid, dim3 block, hipStream_t stream, const BinaryArgs &args)
+{
+    if (layout == OUT_PACKED)
+        hipLaunchKernelGGL((binary_tile_kernel<OUT_PACKED, FASTDIV>), grid, block, 0, stream, args);
+    else if (layout == OUT_FULL)
+        hipLaunchKernelGGL((binary_tile_kernel<OUT_FULL, FASTDIV>), grid, block, 0, stream, args);
+    else
+        hipLaunchKernelGGL((binary_tile_kernel<OUT_CROSS, FASTDIV>), grid, block, 0, stream, args);
 }
 
 }  // namespace
@@ -221,12 +273,11 @@ hipError_t launch_binary(int layout, const uint32_t *a, int64_t na, const uint32
         args.tile_offset = off;
         args.tiles_this = (total - off) < max_tiles ? (total - off) : max_tiles;
         dim3 grid((unsigned)args.tiles_this), block(BTHREADS);
-        if (layout == OUT_PACKED)
-            hipLaunchKernelGGL(binary_tile_kernel<OUT_PACKED>, grid, block, 0, stream, args);
-        else if (layout == OUT_FULL)
-            hipLaunchKernelGGL(binary_tile_kernel<OUT_FULL>, grid, block, 0, stream, args);
+        // the counts are at most 32 x words
+        if (words * 32 <= SMALL_DIV_MAX)
+            launch_layout<true>(layout, grid, block, stream, args);
         else
-            hipLaunchKernelGGL(binary_tile_kernel<OUT_CROSS>, grid, block, 0, stream, args);
+            launch_layout<false>(layout, grid, block, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

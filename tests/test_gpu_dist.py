@@ -110,6 +110,28 @@ def test_many_features(n, k, method):
     assert_parity(run(x, method, p), orc.dist(x, method, p), method)
 
 
+@pytest.mark.parametrize("layout", ["packed", "full"])
+def test_binary_beyond_the_fast_quotient(layout):
+    """Past 131072 features the binary ratio takes the IEEE divide instead of
+    the reciprocal-and-correction form; both are bit-exact."""
+    n, k = 70, 140001
+    x = wl.bernoulli(n, k, seed=23)
+    x[5] = 0.0                          # no bit on in either row of (5, 6): 0
+    x[6] = 0.0
+    print("case binary k beyond fast quotient", layout, flush=True)
+    want = orc.dist(x, "binary")
+    if layout == "packed":
+        assert_parity(run(x, "binary"), want, "binary")
+    else:
+        full = gt.dist(dev(x), "binary", layout="full")
+        torch.cuda.synchronize()
+        m = full.cpu().numpy()
+        ii, jj = np.tril_indices(n, -1)
+        order = np.lexsort((ii, jj))
+        np.testing.assert_array_equal(m[ii[order], jj[order]], want.astype(np.float32))
+        assert m[6, 5] == 0.0 and np.all(np.diag(m) == 0.0)
+
+
 @pytest.mark.parametrize("method", ["euclidean", "manhattan", "minkowski"])
 def test_many_features_minkowski_powers_and_cross(method):
     x = wl.uniform(100, 20000, seed=19)
