@@ -98,7 +98,7 @@ struct GramArgs {
 // Pre-pass: split every centred value x - c once.  Term t of row r, feature
 // c, at terms[(t * n + r) * kp + c]; features k..kp-1 are zero.  One thread
 // per 8 consecutive features (one 16 B store per term).
-template <int SPLIT>
+template <int SPLIT, bool VEC4>
 __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x, int64_t n,
                                                     int64_t k, int64_t ldx, int64_t kp,
                                                     const float *__restrict__ center,
@@ -110,14 +110,34 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
     const int64_t units = n * per_row;
     for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < units;
          u += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = u / per_row, c0 = (u % per_row) * 8;
+        int64_t r, c0;
+        if (units <= 0xffffffffll) {
+            // 32-bit division (the 64-bit one is a long instruction sequence)
+            const uint32_t uu = (uint32_t)u, pr = (uint32_t)per_row;
+            r = uu / pr;
+            c0 = (int64_t)(uu - (uint32_t)r * pr) * 8;
+        } else {
+            r = u / per_row;
+            c0 = (u % per_row) * 8;
+        }
         const float sc = (SPLIT == SPLIT_FP16) ? scales[r] : 1.0f;
+        float v[8];
+        if (VEC4 && c0 + 8 <= k) {
+            const float4 a = *reinterpret_cast<const float4 *>(x + r * ldx + c0);
+            const float4 b = *reinterpret_cast<const float4 *>(x + r * ldx + c0 + 4);
+            const float4 ca = *reinterpret_cast<const float4 *>(center + c0);
+            const float4 cb = *reinterpret_cast<const float4 *>(center + c0 + 4);
+            v[0] = a.x - ca.x; v[1] = a.y - ca.y; v[2] = a.z - ca.z; v[3] = a.w - ca.w;
+            v[4] = b.x - cb.x; v[5] = b.y - cb.y; v[6] = b.z - cb.z; v[7] = b.w - cb.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = (c0 + c < k) ? x[r * ldx + c0 + c] - center[c0 + c] : 0.0f;
+        }
         unsigned short t[8][NT];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            float v = (c0 + c < k) ? x[r * ldx + c0 + c] - center[c0 + c] : 0.0f;
-            if (SPLIT == SPLIT_FP16) v *= sc;
-            Split<SPLIT>::split(v, t[c]);
+            if (SPLIT == SPLIT_FP16) v[c] *= sc;
+            Split<SPLIT>::split(v[c], t[c]);
         }
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
@@ -620,32 +640,55 @@ __global__ __launch_bounds__(256) void colstats_partial_kernel(const float *__re
     }
 }
 
+// Column statistics from the partials: 16 columns per workgroup, 16
+// threads per column, each over every 16th partial block; the 16 partial
+// results are combined in a fixed order.
+constexpr int CS_COLS = 16, CS_SLICES = 16;
+
 __global__ __launch_bounds__(256) void colstats_finish_kernel(const float *__restrict__ psum,
                                                               const float *__restrict__ pmin,
                                                               const float *__restrict__ pmax,
                                                               int64_t blocks, int64_t n, int64_t k,
                                                               float *__restrict__ center)
 {
-    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < k; c += (int64_t)gridDim.x * 256) {
+    __shared__ float red[3][CS_SLICES][CS_COLS];
+    const int cl = threadIdx.x % CS_COLS, sl = threadIdx.x / CS_COLS;
+    for (int64_t cb = (int64_t)blockIdx.x * CS_COLS; cb < k; cb += (int64_t)gridDim.x * CS_COLS) {
+        const int64_t c = cb + cl;
         float sum = 0.0f, mn = __builtin_inff(), mx = -__builtin_inff();
-        for (int64_t b = 0; b < blocks; ++b) {
-            sum += psum[b * k + c];
-            mn = fminf(mn, pmin[b * k + c]);
-            mx = fmaxf(mx, pmax[b * k + c]);
-        }
-        float cc = 0.0f;
-        const float range = mx - mn;
-        if (isfinite(sum) && isfinite(range)) {
-            if (range == 0.0f) {
-                cc = mn;
-            } else {
-                const float mean = sum / (float)n;
-                const float g = ldexpf(1.0f, ilogbf(range) - 6);   // <= range / 64
-                const float q = mean / g;
-                cc = (fabsf(q) < 0x1p24f) ? rintf(q) * g : mean;
+        if (c < k) {
+            for (int64_t b = sl; b < blocks; b += CS_SLICES) {
+                sum += psum[b * k + c];
+                mn = fminf(mn, pmin[b * k + c]);
+                mx = fmaxf(mx, pmax[b * k + c]);
             }
         }
-        center[c] = cc;
+        red[0][sl][cl] = sum;
+        red[1][sl][cl] = mn;
+        red[2][sl][cl] = mx;
+        __syncthreads();
+        if (sl == 0 && c < k) {
+            sum = 0.0f;
+            for (int q = 0; q < CS_SLICES; ++q) {
+                sum += red[0][q][cl];
+                mn = fminf(mn, red[1][q][cl]);
+                mx = fmaxf(mx, red[2][q][cl]);
+            }
+            float cc = 0.0f;
+            const float range = mx - mn;
+            if (isfinite(sum) && isfinite(range)) {
+                if (range == 0.0f) {
+                    cc = mn;
+                } else {
+                    const float mean = sum / (float)n;
+                    const float g = ldexpf(1.0f, ilogbf(range) - 6);   // <= range / 64
+                    const float q = mean / g;
+                    cc = (fabsf(q) < 0x1p24f) ? rintf(q) * g : mean;
+                }
+            }
+            center[c] = cc;
+        }
+        __syncthreads();
     }
 }
 
@@ -730,9 +773,14 @@ hipError_t launch_split(int layout, bool vec4, GramArgs &args, const float *cent
         const int64_t units = args.n * (args.kp / 8);
         int64_t blocks = (units + 255) / 256;
         if (blocks > 65536) blocks = 65536;
-        hipLaunchKernelGGL(split_kernel<SPLIT>, dim3((unsigned)blocks), dim3(256), 0, stream, args.x,
-                           args.n, args.k, args.ldx, args.kp, center, scales,
-                           const_cast<unsigned short *>(args.terms));
+        if (vec4)
+            hipLaunchKernelGGL((split_kernel<SPLIT, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                               args.x, args.n, args.k, args.ldx, args.kp, center, scales,
+                               const_cast<unsigned short *>(args.terms));
+        else
+            hipLaunchKernelGGL((split_kernel<SPLIT, false>), dim3((unsigned)blocks), dim3(256), 0,
+                               stream, args.x, args.n, args.k, args.ldx, args.kp, center, scales,
+                               const_cast<unsigned short *>(args.terms));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -826,8 +874,9 @@ hipError_t launch_gram(int precision, int layout, const float *x, int64_t n, int
                            dim3(256), 0, stream, x, n, k, ldx, psum, pmin, pmax);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(colstats_finish_kernel, dim3((unsigned)(cb < 4096 ? cb : 4096)), dim3(256), 0,
-                           stream, psum, pmin, pmax, csb, n, k, center);
+        const int64_t fb = (k + CS_COLS - 1) / CS_COLS;
+        hipLaunchKernelGGL(colstats_finish_kernel, dim3((unsigned)(fb < 65536 ? fb : 65536)), dim3(256),
+                           0, stream, psum, pmin, pmax, csb, n, k, center);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
