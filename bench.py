@@ -2,9 +2,10 @@
 
 One step = one gt_dist call over the whole point set (inputs resident in HBM,
 packed FP32 lower-triangle output of n(n-1)/2 = 2,147,450,880 distances in
-HBM).  The headline precision (default: BF16 hi+lo split, three MFMA
-products, exact FP32 recompute of near-coincident pairs) and the plain FP32
-VALU kernel are both timed; value is the headline path's pair-distances/s.
+HBM).  The headline precision (default: BF16 three-term split, six MFMA
+products per pair of terms' chunks, exact FP32 recompute of near-coincident
+pairs) and the plain FP32 VALU kernel are both timed; value is the headline
+path's pair-distances/s.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3]
 
@@ -59,6 +60,10 @@ def parse():
                          "profiling them)")
     ap.add_argument("--no-host-path", action="store_true",
                     help="skip timing the host-buffer entry point (PCIe-inclusive; never `value`)")
+    ap.add_argument("--calibrate-peaks", action="store_true",
+                    help="measure sustained FP32 FMA, integer AND/popcount, BF16 MFMA and copy "
+                         "rates with register-only probe kernels (libgt_calib.so) and add them "
+                         "as measured_peaks")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (reference kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -428,6 +433,59 @@ def secondary_paths(torch, gt, wl, select=None, steps=3, warmup=1):
     return out
 
 
+def measure_peaks(torch):
+    """Sustained rates of register-only probe loops (gputools_amd/calib/
+    gt_calib.hip) on this GPU: context for the published peaks the roofline
+    fractions use.  2048 workgroups of 256 threads (8 per CU)."""
+    path = os.path.join(ROOT, "gputools_amd", "libgt_calib.so")
+    if not os.path.exists(path):
+        return {"skipped": "libgt_calib.so not built"}
+    L = ctypes.CDLL(path)
+    vp, cf = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)
+    for name in ("gt_calib_fma", "gt_calib_popc", "gt_calib_mfma"):
+        getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, cf]
+        getattr(L, name).restype = ctypes.c_int
+    L.gt_calib_copy.argtypes = [vp, vp, ctypes.c_int64, vp, cf]
+    L.gt_calib_copy.restype = ctypes.c_int
+    stream = torch.cuda.current_stream().cuda_stream or None
+    blocks, threads = 2048, 256
+    buf = torch.empty(blocks * threads, dtype=torch.float32, device="cuda")
+    ms = ctypes.c_float(0.0)
+
+    def run(fn, *a):
+        for _ in range(2):          # the first launch loads the code object
+            if fn(*a, stream, ctypes.byref(ms)) != 0:
+                raise RuntimeError("calibration probe failed")
+        return ms.value
+
+    out = {"probe": "gputools_amd/calib/gt_calib.hip, %d x %d threads" % (blocks, threads)}
+    it = 20000
+    t = run(L.gt_calib_fma, blocks, it, buf.data_ptr())
+    lanes = blocks * threads * it * 16.0
+    out["fp32_fma"] = {"ms": round(t, 3), "tflops": round(2 * lanes / (t * 1e-3) / 1e12, 1),
+                       "t_lane_instr_per_s": round(lanes / (t * 1e-3) / 1e12, 1),
+                       "what": "16 independent non-packed v_fma_f32 chains per lane"}
+    t = run(L.gt_calib_popc, blocks, it, buf.data_ptr())
+    ops = blocks * threads * it * 32.0
+    out["int32_and_popcount"] = {"ms": round(t, 3),
+                                 "t_lane_ops_per_s": round(ops / (t * 1e-3) / 1e12, 1),
+                                 "what": "16 independent v_and_b32 + v_bcnt_u32_b32 chains"}
+    it = 200
+    t = run(L.gt_calib_mfma, blocks, it, buf.data_ptr())
+    flops = blocks * (threads // 64) * it * 8 * 32768.0
+    out["bf16_mfma"] = {"ms": round(t, 3), "pflops": round(flops / (t * 1e-3) / 1e15, 3),
+                        "what": "8 independent v_mfma_f32_32x32x16_bf16 chains per wave"}
+    del buf
+    n4 = (1 << 30) // 16
+    a = torch.ones(n4 * 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    t = run(L.gt_calib_copy, a.data_ptr(), b.data_ptr(), n4)
+    out["copy"] = {"ms": round(t, 3), "gbs_read_plus_write": round(2 * 16.0 * n4 / (t * 1e-3) / 1e9, 1),
+                   "what": "1 GiB float4 device-to-device copy"}
+    del a, b
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -542,6 +600,9 @@ def main():
                    "precision": args.precision, "pairs": npairs},
         "roofline": roof,
     }
+    if args.calibrate_peaks:
+        print("bench: calibration probes", file=sys.stderr, flush=True)
+        line["measured_peaks"] = measure_peaks(torch)
     if args.precision != "fp32" and "fp32" in results:
         f = results["fp32"]
         line["fp32_plain"] = {"ms_per_step": round(f["ms_per_step"], 4),

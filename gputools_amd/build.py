@@ -17,6 +17,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libgputools_amd.so")
+# Peak-rate probes for the measurement figures (bench.py --calibrate-peaks);
+# a library of their own, not linked into the distance library.
+CALIB_SRC = os.path.join(HERE, "calib", "gt_calib.hip")
+CALIB_LIB = os.path.join(HERE, "libgt_calib.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
@@ -59,9 +63,20 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    build_calib()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CALIB_LIB)
     return LIB
+
+
+def build_calib() -> str:
+    tmp = CALIB_LIB + ".tmp"
+    cmd = [HIPCC] + COMMON + DEVICE + ["-shared", CALIB_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed on %s:\n%s\n%s" % (CALIB_SRC, " ".join(cmd), r.stderr))
+    os.replace(tmp, CALIB_LIB)
+    return CALIB_LIB
 
 
 if __name__ == "__main__":
