@@ -240,6 +240,35 @@ def test_gpucor_larger_self_sampled_rows():
     np.testing.assert_array_equal(res["coefficients"], res["coefficients"].T)
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("self_case,na,nb,dim", [(True, 5788, None, 300),
+                                                 (False, 4302, 4089, 200)])
+def test_device_pmcc_larger_ragged(precision, self_case, na, nb, dim):
+    """More tiles than CUs with a short last round (276 = 256 + 20 tiles
+    symmetric, 17 x 16 = 272 = 256 + 16 cross), ragged edges: the bottom two
+    tile rows (the last tiles in schedule order) against every column, and
+    the first 64 rows."""
+    rng = np.random.default_rng(24)
+    a = _data(rng, dim, na, shift=0.25).T.copy()
+    b = None if self_case else _data(rng, dim, nb, shift=-0.5).T.copy()
+    print("case pmcc larger ragged %s self=%s %d %s %d" % (precision, self_case, na, nb, dim), flush=True)
+    pairs, corr, ts = gt.pmcc(torch.from_numpy(a).cuda(),
+                              None if self_case else torch.from_numpy(b).cuda(), precision=precision)
+    torch.cuda.synchronize()
+    got = corr.cpu().numpy()
+    other = a if self_case else b
+    x0 = (na - 1) // 256 * 256 - 256           # the last two tile rows
+    for rows in (np.arange(x0, na), np.arange(0, 64)):
+        _, ref = orc.pmcc(a[rows], other, nthreads=16)
+        err = np.abs(got[rows] - ref).max()
+        print("rows %d..%d max abs error %.3g" % (rows[0], rows[-1], err), flush=True)
+        assert err <= TOL
+        assert_bits_equal(ts.cpu().numpy()[rows], orc.signif(pairs.cpu().numpy()[rows], got[rows]))
+    assert (pairs.cpu().numpy() == dim).all()
+    if self_case:
+        np.testing.assert_array_equal(got, got.T)
+
+
 def test_gputtest_matches_gpusignif():
     rng = np.random.default_rng(51)
     r = np.concatenate([rng.uniform(-1, 1, 5000), [0.9995, 1.0, -1.0, 0.0, -0.0, 2.0, np.nan]])
