@@ -7,6 +7,9 @@
 // tile kernel counts popcount(a & b) per pair and word (one AND and one
 // v_bcnt_u32_b32, which adds to an accumulator).  From the row counts pa, pb:
 //   on in exactly one row = pa + pb - 2 (a & b),  on in either = pa + pb - (a & b).
+// Packed requests on one point set take the matrix cores instead: one 0/1
+// byte per feature and v_mfma_i32_32x32x32_i8 give the same exact count of
+// features on in both rows (binary_mfma_kernel).
 // The counts are exact integers; their ratio is rounded correctly to FP32,
 // which equals the FP32 rounding of the FP64 ratio R computes (double
 // rounding of a quotient from binary64 to binary32 is innocuous), so the
@@ -224,6 +227,225 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packed whole-matrix requests: the counts on the matrix cores.  Each row is
+// one 0/1 byte per feature (zero-padded to a multiple of 32) and its count of
+// ones; v_mfma_i32_32x32x32_i8 sums the products of two rows' bytes exactly,
+// which is the number of features on in both.  The tiles are 256 x 256 on 16
+// waves (64 x 64 pairs each); chunks of 32 features (32 B per row) reach LDS
+// through global_load_lds into a three-stage ring, 16-byte halves swapped by
+// ((R >> 2) ^ (R >> 3)) & 1 so that the operand reads of consecutive rows
+// hit distinct banks.
+// ---------------------------------------------------------------------------
+typedef int bin_i32x4_t __attribute__((ext_vector_type(4)));
+typedef int bin_i32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int QBK = 32;                  // features per chunk (bytes per row)
+constexpr int QNSTAGE = 3;
+constexpr int QSUPER = 4;                // super-tile side in tiles
+// Tile side: 256 x 256 pairs (16 waves) for large n; 128 x 128 (4 waves,
+// four workgroups per CU) while the 256 schedule has fewer than QSMALL_ROUNDS
+// rounds of 256 CUs (65536 x 256: 2.73 vs 3.07 ms; 8192 x 256: 0.049 vs
+// 0.043 ms).
+constexpr int64_t QSMALL_ROUNDS = 4;
+
+// Tile geometry: QBM x QBM pairs on (QBM / 64)^2 waves of 64 x 64.
+template <int QBM>
+struct QTile {
+    static constexpr int WR = QBM / 64;            // waves per side
+    static constexpr int NW = WR * WR;
+    static constexpr int THREADS = NW * 64;
+    static constexpr int TERM = QBM * QBK;         // one side of one chunk
+    static constexpr int STAGE = 2 * TERM;
+    static constexpr int GPW = 2 * (QBM / 32) / NW;   // 32-row load groups per wave
+};
+
+int64_t bytes_ld(int64_t k) { return (k + QBK - 1) / QBK * QBK; }
+
+// One wave per row: bytes (x != 0) of 4 features per lane, and the row's
+// number of ones.
+__global__ __launch_bounds__(256) void binary_pack8_kernel(const float *__restrict__ x, int64_t n,
+                                                           int64_t k, int64_t ldx, int64_t kb,
+                                                           uint32_t *__restrict__ ind,
+                                                           uint32_t *__restrict__ cnt)
+{
+    const int lane = threadIdx.x & 63;
+    for (int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < n;
+         row += (int64_t)gridDim.x * 4) {
+        const float *xr = x + row * ldx;
+        uint32_t *ir = ind + row * (kb / 4);
+        uint32_t ones = 0;
+        for (int64_t f0 = 0; f0 < kb; f0 += 256) {
+            const int64_t f = f0 + 4 * lane;
+            if (f >= kb) break;
+            uint32_t w = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool on = (f + e < k) && (xr[f + e] != 0.0f);
+                w |= (on ? 1u : 0u) << (8 * e);
+                ones += on ? 1u : 0u;
+            }
+            ir[f / 4] = w;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) ones += __shfl_xor(ones, d, 64);
+        if (lane == 0) cnt[row] = ones;
+    }
+}
+
+struct BinMfmaArgs {
+    const unsigned char *ind;   // n x kb bytes
+    const uint32_t *cnt;        // ones per row
+    int64_t n, kb;
+    float *out;                 // packed
+    int64_t T;                  // tiles per side
+    int64_t id_offset, ids_this;
+};
+
+template <int QBM, bool FASTDIV>
+__global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) void binary_mfma_kernel(
+    BinMfmaArgs args)
+{
+    typedef QTile<QBM> Q;
+    constexpr int QTERM = Q::TERM, QSTAGE = Q::STAGE, NW = Q::NW, GPW = Q::GPW;
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[QNSTAGE * QSTAGE + 2 * QBM * 4];
+    uint32_t *cA = reinterpret_cast<uint32_t *>(smem + QNSTAGE * QSTAGE);
+    uint32_t *cB = cA + QBM;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave / Q::WR, wc = wave % Q::WR;   // A rows (j) wr*64.., B rows (i) wc*64..
+
+    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int64_t st = id / (QSUPER * QSUPER);
+    const int w = (int)(id % (QSUPER * QSUPER));
+    int SI, SJ;
+    tri_tile(st, SI, SJ);
+    const int64_t I = (int64_t)SI * QSUPER + w / QSUPER;
+    const int64_t J = (int64_t)SJ * QSUPER + w % QSUPER;
+    if (I >= args.T || J > I) return;
+    const int64_t i0 = I * QBM, j0 = J * QBM, n = args.n;
+    for (int t = tid; t < 2 * QBM; t += Q::THREADS) {
+        if (t < QBM)
+            cA[t] = j0 + t < n ? args.cnt[j0 + t] : 0u;
+        else
+            cB[t - QBM] = i0 + t - QBM < n ? args.cnt[i0 + t - QBM] : 0u;
+    }
+
+    // the 2 QBM / 32 groups of 32 rows go to the waves round robin: wave w
+    // fills groups w, w + NW, ... (group g: side g / (QBM / 32), rows
+    // 32 (g % (QBM / 32)) ..); rows past n are clamped to n - 1 (they only
+    // feed pairs never stored)
+    typedef __attribute__((address_space(3))) void lds_void_t;
+    typedef const __attribute__((address_space(1))) void gbl_void_t;
+    constexpr int GSIDE = QBM / 32;
+    const unsigned char *gsrc[GPW];
+    unsigned char *ldst[GPW];
+#pragma unroll
+    for (int q = 0; q < GPW; ++q) {
+        const int g = wave + q * NW;
+        const int rr = lane >> 1;
+        const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
+        const int64_t r = ((g / GSIDE) ? i0 : j0) + (g % GSIDE) * 32 + rr;
+        gsrc[q] = args.ind + (r < n ? r : n - 1) * args.kb + half * 16;
+        ldst[q] = smem + (g / GSIDE) * QTERM + (g % GSIDE) * 1024;
+    }
+    auto issue = [&](int chunk) {
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+            __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[q] + (int64_t)chunk * QBK),
+                                             (lds_void_t *)(ldst[q] + (chunk % QNSTAGE) * QSTAGE), 16,
+                                             0, 0);
+    };
+    const int lrow = lane & 31;
+    const int roff = lrow * QBK + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
+
+    bin_i32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+
+    const int nchunks = (int)(args.kb / QBK);
+    issue(0);
+    if (nchunks > 1) issue(1);
+    for (int c = 0; c < nchunks; ++c) {
+        // chunk c landed when at most the next chunk's loads of this wave
+        // are outstanding (vector memory loads complete in order)
+        if (c + 1 < nchunks) {
+            if (GPW == 1)
+                asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+            else
+                asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+        } else {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (c + 2 < nchunks) issue(c + 2);
+        const unsigned char *sg = smem + (c % QNSTAGE) * QSTAGE + roff;
+        bin_i32x4_t av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            av[a] = *reinterpret_cast<const bin_i32x4_t *>(sg + (wr * 64 + a * 32) * QBK);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            bv[b] = *reinterpret_cast<const bin_i32x4_t *>(sg + QTERM + (wc * 64 + b * 32) * QBK);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();   // cA / cB visible
+
+    // 32x32 accumulator layout: element r of lane l is A row 8(r/4) +
+    // 4(l/32) + r%4 (j) and B row l%32 (i): lanes 0..31 store 32 consecutive
+    // i of one j.
+    const int li0 = wc * 64 + (lane & 31);
+    uint32_t pb[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) pb[b] = cB[li0 + b * 32];
+    if (I > J && i0 + QBM <= n) {
+        // every pair valid, i > j: row offsets from the wave's first row jw
+        // fit in 32 bits (n < 2^24)
+        const int64_t jw = j0 + wr * 64;
+        float *const outw = args.out + packed_base(n, jw) + i0;
+        const unsigned cw = (unsigned)(n - 1 - jw);
+        const bool hi = (lane >> 5) != 0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const unsigned dc = a * 32 + 8 * (r >> 2) + (r & 3);
+                const unsigned base_u = dc * cw - dc * (dc + 1) / 2;
+                const unsigned step_u = 4 * cw - 4 * dc - 10;
+                const unsigned ro = base_u + (hi ? step_u : 0u) + (unsigned)li0;
+                const uint32_t pa = cA[wr * 64 + (int)dc + (hi ? 4 : 0)];
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    outw[ro + (unsigned)(b * 32)] = binary_value<FASTDIV>(pa + pb[b], (uint32_t)acc[a][b][r]);
+            }
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int64_t j = j0 + lj;
+            if (j >= n) continue;
+            const uint32_t pa = cA[lj];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int64_t i = i0 + li0 + b * 32;
+                if (i < n && i > j)
+                    args.out[packed_base(n, j) + i] = binary_value<FASTDIV>(pa + pb[b], (uint32_t)acc[a][b][r]);
+            }
+        }
+}
+
 template <bool FASTDIV>
 void launch_layout(int layout, dim3 grid, dim3 block, hipStream_t stream, const BinaryArgs &args)
 {
@@ -235,9 +457,77 @@ void launch_layout(int layout, dim3 grid, dim3 block, hipStream_t stream, const 
         hipLaunchKernelGGL((binary_tile_kernel<OUT_CROSS, FASTDIV>), grid, block, 0, stream, args);
 }
 
+template <int QBM>
+hipError_t launch_mfma_tiles(BinMfmaArgs &args, int64_t k, hipStream_t stream)
+{
+    args.T = (args.n + QBM - 1) / QBM;
+    const int64_t S = (args.T + QSUPER - 1) / QSUPER;
+    const int64_t ids = S * (S + 1) / 2 * QSUPER * QSUPER;
+    const int64_t max_ids = int64_t(1) << 23;
+    for (int64_t off = 0; off < ids; off += max_ids) {
+        args.id_offset = off;
+        args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
+        if (k <= SMALL_DIV_MAX)
+            hipLaunchKernelGGL((binary_mfma_kernel<QBM, true>), dim3((unsigned)args.ids_this),
+                               dim3(QTile<QBM>::THREADS), 0, stream, args);
+        else
+            hipLaunchKernelGGL((binary_mfma_kernel<QBM, false>), dim3((unsigned)args.ids_this),
+                               dim3(QTile<QBM>::THREADS), 0, stream, args);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 int64_t binary_words(int64_t k) { return (k + 31) / 32; }
+
+// Workspace of a binary call on one point set: the bit rows (with their
+// counts), then the byte rows and counts of the matrix-core form.
+size_t binary_workspace_bytes(int64_t n, int64_t k)
+{
+    auto a256 = [](size_t v) { return (v + 255) / 256 * 256; };
+    return a256((size_t)n * (size_t)binary_ldw(k) * sizeof(uint32_t)) +
+           a256((size_t)n * (size_t)bytes_ld(k)) + a256((size_t)n * sizeof(uint32_t));
+}
+
+hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, int64_t ldx,
+                              void *workspace, float *out, int64_t ldd, hipStream_t stream)
+{
+    auto a256 = [](size_t v) { return (v + 255) / 256 * 256; };
+    char *ws = static_cast<char *>(workspace);
+    uint32_t *bits = reinterpret_cast<uint32_t *>(ws);
+    const int64_t words = binary_words(k), ldw = binary_ldw(k);
+    if (layout != OUT_PACKED || n >= (int64_t(1) << 24)) {
+        hipError_t e = launch_binary_pack(x, n, k, ldx, bits, ldw, stream);
+        if (e != hipSuccess) return e;
+        return launch_binary(layout, bits, n, bits, n, words, ldw, out, ldd, stream);
+    }
+    const int64_t kb = bytes_ld(k);
+    ws += a256((size_t)n * (size_t)ldw * sizeof(uint32_t));
+    unsigned char *ind = reinterpret_cast<unsigned char *>(ws);
+    ws += a256((size_t)n * (size_t)kb);
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(ws);
+    {
+        int64_t blocks = (n + 3) / 4;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(binary_pack8_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, n, k,
+                           ldx, kb, reinterpret_cast<uint32_t *>(ind), cnt);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    BinMfmaArgs args;
+    args.ind = ind;
+    args.cnt = cnt;
+    args.n = n;
+    args.kb = kb;
+    args.out = out;
+    const int64_t t256 = (n + 255) / 256;
+    if (t256 * (t256 + 1) / 2 < QSMALL_ROUNDS * 256)
+        return launch_mfma_tiles<128>(args, k, stream);
+    return launch_mfma_tiles<256>(args, k, stream);
+}
 
 int64_t binary_ldw(int64_t k) { return binary_words(k) + 1; }
 

@@ -104,7 +104,7 @@ int workspace_bytes(int64_t n, int64_t k, int method, int precision, size_t *byt
     size_t b = 0;
     if (k > 0 && n > 1) {
         if (method == GT_BINARY)
-            b = round_up((size_t)n * (size_t)gt::binary_ldw(k) * sizeof(uint32_t), 256);
+            b = gt::binary_workspace_bytes(n, k);
         else if (precision != GT_PREC_FP32)
             b = gt::gram_workspace_bytes(n, k, precision);
     }
@@ -161,10 +161,7 @@ int run_dist(const float *x, int64_t n, int64_t k, int64_t ldx, int method, floa
             GT_HIP(gt::launch_fill_full(out, 1, ldd, 0.0f, s));
             return GT_OK;
         }
-        uint32_t *bits = static_cast<uint32_t *>(ws);
-        const int64_t words = gt::binary_words(k), ldw = gt::binary_ldw(k);
-        GT_HIP(gt::launch_binary_pack(x, n, k, ldx, bits, ldw, s));
-        GT_HIP(gt::launch_binary(glayout, bits, n, bits, n, words, ldw, out, ldd, s));
+        GT_HIP(gt::launch_binary_self(glayout, x, n, k, ldx, ws, out, ldd, s));
         return GT_OK;
     }
     if (precision != GT_PREC_FP32) {

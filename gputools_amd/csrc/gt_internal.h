@@ -39,6 +39,11 @@ hipError_t launch_binary_pack(const float *x, int64_t n, int64_t k, int64_t ldx,
 hipError_t launch_binary(int layout, const uint32_t *a, int64_t na, const uint32_t *b,
                          int64_t nb, int64_t words, int64_t ldw, float *out, int64_t ldd,
                          hipStream_t stream);
+// One point set (the self distances): packed layouts below 2^24 points count
+// on the matrix cores (one byte per feature), the others on the bit rows.
+size_t binary_workspace_bytes(int64_t n, int64_t k);
+hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, int64_t ldx,
+                              void *workspace, float *out, int64_t ldd, hipStream_t stream);
 
 // Fill n floats with a value (used for the k == 0 case: every distance NA).
 hipError_t launch_fill(float *out, int64_t count, float value, hipStream_t stream);

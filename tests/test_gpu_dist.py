@@ -110,6 +110,28 @@ def test_many_features(n, k, method):
     assert_parity(run(x, method, p), orc.dist(x, method, p), method)
 
 
+def test_binary_matrix_core_large_tiles():
+    """12000 points: the packed binary path counts on the matrix cores in
+    256 x 256 tiles (fewer points use 128 x 128 tiles).  Whole rows across
+    the tile edges and random pairs, bit-exact."""
+    n, k = 12000, 100
+    x = wl.bernoulli(n, k, seed=25)
+    x[300] = 0.0
+    print("case binary matrix-core 12000x100", flush=True)
+    got = gt.dist(dev(x), "binary")
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(3)
+    jj = np.concatenate([np.repeat([0, 255, 256, 300, 11775, 11998], n),
+                         rng.integers(0, n, 200000)])
+    ii = np.concatenate([np.tile(np.arange(n), 6), rng.integers(0, n, 200000)])
+    keep = ii != jj
+    ii, jj = ii[keep], jj[keep]
+    hi, lo = np.maximum(ii, jj), np.minimum(ii, jj)
+    idx = lo * n - lo * (lo + 1) // 2 + (hi - lo - 1)
+    vals = got[torch.from_numpy(idx).cuda()].cpu().numpy()
+    np.testing.assert_array_equal(vals, orc.dist_pairs(x, hi, lo, "binary").astype(np.float32))
+
+
 @pytest.mark.parametrize("layout", ["packed", "full"])
 def test_binary_beyond_the_fast_quotient(layout):
     """Past 131072 features the binary ratio takes the IEEE divide instead of
