@@ -60,6 +60,7 @@ CASES = [
     ("euclidean", "fp16x3", "packed", (1500, 300), (333, 17)),
     ("euclidean", "bf16x3", "full", (900, 130), (257, 64)),
     ("binary", "fp32", "packed", (3000, 1000), (1000, 77)),
+    ("binary", "fp32", "full", (700, 300), (300, 65)),
     ("euclidean", "fp32", "packed", (1200, 64), (300, 33)),
     # near the scratch limit: 3 x 8192 x 1024 BF16 terms are 48 MiB
     ("euclidean", "bf16x3", "packed", (8192, 1024), (4097, 512)),
