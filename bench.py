@@ -447,6 +447,8 @@ def measure_peaks(torch):
         getattr(L, name).restype = ctypes.c_int
     L.gt_calib_copy.argtypes = [vp, vp, ctypes.c_int64, vp, cf]
     L.gt_calib_copy.restype = ctypes.c_int
+    L.gt_calib_fill.argtypes = [vp, ctypes.c_int64, vp, cf]
+    L.gt_calib_fill.restype = ctypes.c_int
     stream = torch.cuda.current_stream().cuda_stream or None
     blocks, threads = 2048, 256
     buf = torch.empty(blocks * threads, dtype=torch.float32, device="cuda")
@@ -483,6 +485,12 @@ def measure_peaks(torch):
     out["copy"] = {"ms": round(t, 3), "gbs_read_plus_write": round(2 * 16.0 * n4 / (t * 1e-3) / 1e9, 1),
                    "what": "1 GiB float4 device-to-device copy"}
     del a, b
+    n4 = (8 << 30) // 16
+    a = torch.empty(n4 * 4, dtype=torch.float32, device="cuda")
+    t = run(L.gt_calib_fill, a.data_ptr(), n4)
+    out["fill"] = {"ms": round(t, 3), "gbs_write": round(16.0 * n4 / (t * 1e-3) / 1e9, 1),
+                   "what": "8 GiB float4 write-only fill (the size of the packed 65536-point output)"}
+    del a
     return out
 
 

@@ -1,7 +1,7 @@
 // gt_calib.hip -- peak-rate probes behind the measured figures quoted next to
 // the published MI355X peaks: register-only loops of independent FP32 FMAs,
-// of 32-bit AND + popcount-accumulate, and of BF16 MFMAs, and a streaming
-// copy.  Built into its own library (libgt_calib.so); the distance library
+// of 32-bit AND + popcount-accumulate, and of BF16 MFMAs, a streaming copy
+// and a write-only fill.  Built into its own library (libgt_calib.so); the distance library
 // does not use it.  Each probe returns the kernel time of one launch in ms
 // (hipEvent pair on the given stream), and the caller derives the rate from
 // the work it states.
@@ -93,6 +93,14 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ sr
         dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void fill_kernel(float4 *__restrict__ dst, int64_t n4, float v)
+{
+    const float4 w = make_float4(v, v, v, v);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = w;
+}
+
 template <typename Launch>
 int timed(hipStream_t stream, float *ms, Launch launch)
 {
@@ -148,6 +156,15 @@ int gt_calib_copy(const void *src, void *dst, int64_t n4, hipStream_t stream, fl
     return timed(stream, ms, [&] {
         hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, stream,
                            static_cast<const float4 *>(src), static_cast<float4 *>(dst), n4);
+    });
+}
+
+// Writes n4 float4 values; traffic 16 * n4 bytes (write only).
+int gt_calib_fill(void *dst, int64_t n4, hipStream_t stream, float *ms)
+{
+    return timed(stream, ms, [&] {
+        hipLaunchKernelGGL(fill_kernel, dim3(8192), dim3(256), 0, stream, static_cast<float4 *>(dst), n4,
+                           1.0f);
     });
 }
 
