@@ -61,9 +61,10 @@ def parse():
     ap.add_argument("--no-host-path", action="store_true",
                     help="skip timing the host-buffer entry point (PCIe-inclusive; never `value`)")
     ap.add_argument("--calibrate-peaks", action="store_true",
-                    help="measure sustained FP32 FMA, integer AND/popcount, BF16 MFMA and copy "
-                         "rates with register-only probe kernels (libgt_calib.so) and add them "
-                         "as measured_peaks")
+                    help="(default) measure sustained FP32 FMA, integer AND/popcount, BF16 MFMA, "
+                         "copy and write-only fill rates with probe kernels (libgt_calib.so), "
+                         "after the timed steps, and add them as measured_peaks")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the measured_peaks probes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (reference kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -608,7 +609,7 @@ def main():
                    "precision": args.precision, "pairs": npairs},
         "roofline": roof,
     }
-    if args.calibrate_peaks:
+    if args.calibrate_peaks or not args.no_peaks:
         print("bench: calibration probes", file=sys.stderr, flush=True)
         line["measured_peaks"] = measure_peaks(torch)
     if args.precision != "fp32" and "fp32" in results:
