@@ -7,9 +7,9 @@
 // tile kernel counts popcount(a & b) per pair and word (one AND and one
 // v_bcnt_u32_b32, which adds to an accumulator).  From the row counts pa, pb:
 //   on in exactly one row = pa + pb - 2 (a & b),  on in either = pa + pb - (a & b).
-// Packed requests on one point set take the matrix cores instead: one 0/1
-// byte per feature and v_mfma_i32_32x32x32_i8 give the same exact count of
-// features on in both rows (binary_mfma_kernel).
+// Packed requests on one point set of up to SMALL_DIV_MAX features take the
+// matrix cores instead: one 0/1 byte per feature and v_mfma_i32_32x32x32_i8
+// give the same exact count of features on in both rows (binary_mfma_kernel).
 // The counts are exact integers; their ratio is rounded correctly to FP32,
 // which equals the FP32 rounding of the FP64 ratio R computes (double
 // rounding of a quotient from binary64 to binary32 is innocuous), so the
@@ -302,7 +302,9 @@ struct BinMfmaArgs {
     int64_t id_offset, ids_this;
 };
 
-template <int QBM, bool FASTDIV>
+// Only called with at most SMALL_DIV_MAX features: the quotient is
+// small_quotient's.
+template <int QBM>
 __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) void binary_mfma_kernel(
     BinMfmaArgs args)
 {
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
                 const uint32_t pa = cA[wr * 64 + (int)dc + (hi ? 4 : 0)];
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                    outw[ro + (unsigned)(b * 32)] = binary_value<FASTDIV>(pa + pb[b], (uint32_t)acc[a][b][r]);
+                    outw[ro + (unsigned)(b * 32)] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
             }
         return;
     }
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
             for (int b = 0; b < 2; ++b) {
                 const int64_t i = i0 + li0 + b * 32;
                 if (i < n && i > j)
-                    args.out[packed_base(n, j) + i] = binary_value<FASTDIV>(pa + pb[b], (uint32_t)acc[a][b][r]);
+                    args.out[packed_base(n, j) + i] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
             }
         }
 }
@@ -458,7 +460,7 @@ void launch_layout(int layout, dim3 grid, dim3 block, hipStream_t stream, const 
 }
 
 template <int QBM>
-hipError_t launch_mfma_tiles(BinMfmaArgs &args, int64_t k, hipStream_t stream)
+hipError_t launch_mfma_tiles(BinMfmaArgs &args, hipStream_t stream)
 {
     args.T = (args.n + QBM - 1) / QBM;
     const int64_t S = (args.T + QSUPER - 1) / QSUPER;
@@ -467,12 +469,8 @@ hipError_t launch_mfma_tiles(BinMfmaArgs &args, int64_t k, hipStream_t stream)
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
         args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
-        if (k <= SMALL_DIV_MAX)
-            hipLaunchKernelGGL((binary_mfma_kernel<QBM, true>), dim3((unsigned)args.ids_this),
-                               dim3(QTile<QBM>::THREADS), 0, stream, args);
-        else
-            hipLaunchKernelGGL((binary_mfma_kernel<QBM, false>), dim3((unsigned)args.ids_this),
-                               dim3(QTile<QBM>::THREADS), 0, stream, args);
+        hipLaunchKernelGGL((binary_mfma_kernel<QBM>), dim3((unsigned)args.ids_this),
+                           dim3(QTile<QBM>::THREADS), 0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -484,12 +482,14 @@ hipError_t launch_mfma_tiles(BinMfmaArgs &args, int64_t k, hipStream_t stream)
 int64_t binary_words(int64_t k) { return (k + 31) / 32; }
 
 // Workspace of a binary call on one point set: the bit rows (with their
-// counts), then the byte rows and counts of the matrix-core form.
+// counts), then, up to SMALL_DIV_MAX features, the byte rows and counts of
+// the matrix-core form (a quarter of the input's size).
 size_t binary_workspace_bytes(int64_t n, int64_t k)
 {
     auto a256 = [](size_t v) { return (v + 255) / 256 * 256; };
-    return a256((size_t)n * (size_t)binary_ldw(k) * sizeof(uint32_t)) +
-           a256((size_t)n * (size_t)bytes_ld(k)) + a256((size_t)n * sizeof(uint32_t));
+    size_t b = a256((size_t)n * (size_t)binary_ldw(k) * sizeof(uint32_t));
+    if (k <= SMALL_DIV_MAX) b += a256((size_t)n * (size_t)bytes_ld(k)) + a256((size_t)n * sizeof(uint32_t));
+    return b;
 }
 
 hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, int64_t ldx,
@@ -499,7 +499,7 @@ hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, 
     char *ws = static_cast<char *>(workspace);
     uint32_t *bits = reinterpret_cast<uint32_t *>(ws);
     const int64_t words = binary_words(k), ldw = binary_ldw(k);
-    if (layout != OUT_PACKED || n >= (int64_t(1) << 24)) {
+    if (layout != OUT_PACKED || n >= (int64_t(1) << 24) || k > SMALL_DIV_MAX) {
         hipError_t e = launch_binary_pack(x, n, k, ldx, bits, ldw, stream);
         if (e != hipSuccess) return e;
         return launch_binary(layout, bits, n, bits, n, words, ldw, out, ldd, stream);
@@ -525,8 +525,8 @@ hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, 
     args.out = out;
     const int64_t t256 = (n + 255) / 256;
     if (t256 * (t256 + 1) / 2 < QSMALL_ROUNDS * 256)
-        return launch_mfma_tiles<128>(args, k, stream);
-    return launch_mfma_tiles<256>(args, k, stream);
+        return launch_mfma_tiles<128>(args, stream);
+    return launch_mfma_tiles<256>(args, stream);
 }
 
 int64_t binary_ldw(int64_t k) { return binary_words(k) + 1; }
