@@ -611,7 +611,11 @@ def main():
     }
     if args.calibrate_peaks or not args.no_peaks:
         print("bench: calibration probes", file=sys.stderr, flush=True)
-        line["measured_peaks"] = measure_peaks(torch)
+        try:
+            line["measured_peaks"] = measure_peaks(torch)
+        except (RuntimeError, OSError) as e:   # torch's out-of-memory error is a RuntimeError
+            line["measured_peaks"] = {"skipped": str(e)[:200]}
+            torch.cuda.empty_cache()
     if args.precision != "fp32" and "fp32" in results:
         f = results["fp32"]
         line["fp32_plain"] = {"ms_per_step": round(f["ms_per_step"], 4),
