@@ -88,3 +88,31 @@ def test_triangle_inequality(method, precision):
     i, j, k = (rng.integers(0, N, 200000) for _ in range(3))
     slack = 4e-6 * (d[i, j] + d[j, k]) + 1e-30
     assert np.all(d[i, k] <= d[i, j] + d[j, k] + slack)
+
+
+@pytest.mark.parametrize("method,precision", CASES)
+def test_known_answers(method, precision):
+    """Unit vectors e_i, e_j (i != j) of k = 64 features: euclidean sqrt 2,
+    maximum 1, manhattan 2, minkowski 2^(1/p), binary 1, canberra k (R drops
+    the 0/0 terms and rescales by k / 2); a repeated point is at 0; x and
+    x + c 1 (c = 0.5, all features) are |c| sqrt k apart in euclidean, |c| in
+    maximum and |c| k in manhattan."""
+    k = 64
+    x = np.vstack([np.eye(k, dtype=np.float32), np.eye(k, dtype=np.float32)[:1]])   # row 64 = e_0
+    d = full(x, method, precision)
+    want = {"euclidean": np.sqrt(np.float32(2.0)), "maximum": 1.0, "manhattan": 2.0,
+            "binary": 1.0, "canberra": float(k), "minkowski": 2.0 ** (1.0 / 3.0)}[method]
+    off = ~np.eye(k + 1, dtype=bool)
+    off[0, k] = off[k, 0] = False
+    vals = d[off]
+    if method == "minkowski":
+        np.testing.assert_allclose(vals, want, rtol=1e-6)
+    else:
+        assert np.all(vals == np.float32(want)), (method, np.unique(vals)[:5])
+    assert d[0, k] == 0.0 and d[k, 0] == 0.0
+    if method in ("euclidean", "maximum", "manhattan"):
+        y = wl.normal(2, k, seed=66)
+        y[1] = y[0] + np.float32(0.5)
+        d2 = full(y, method, precision)[1, 0]
+        expect = {"euclidean": 0.5 * np.sqrt(k), "maximum": 0.5, "manhattan": 0.5 * k}[method]
+        assert abs(float(d2) - expect) <= 1e-6 * expect, (d2, expect)
