@@ -201,3 +201,17 @@ def test_single_linkage_heights_are_the_mst_weights(n):
     assert (sub < sup).all()
     np.testing.assert_array_equal(np.sort(h), want)
     assert (np.diff(h) >= 0).all()          # single linkage heights never decrease
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("state", ["lds", "global"])
+def test_hclust_power_of_two_scaling(method, state):
+    """Every Lance-Williams update is linear in the distances with weights
+    that do not depend on them, so clustering 8 D merges the same clusters
+    at exactly 8 times the heights (FP32, bit for bit)."""
+    d = full_matrix(np.random.default_rng(68).standard_normal((300, 40)).astype(np.float32))
+    sub0, sup0, h0 = run_gpu(d, method, state)
+    sub1, sup1, h1 = run_gpu(d * np.float32(8.0), method, state)
+    np.testing.assert_array_equal(sub1, sub0)
+    np.testing.assert_array_equal(sup1, sup0)
+    np.testing.assert_array_equal(h1, h0 * np.float32(8.0))

@@ -176,3 +176,22 @@ def gt_kendall_ws_bytes(xa, xb):
     gt.lib().gt_kendall_workspace_size(na, na if xb is None else xb.shape[0], n,
                                        1 if xb is None else 0, ctypes.byref(nb))
     return nb.value
+
+
+def test_kendall_invariant_under_monotone_maps():
+    """Kendall's tau depends on the orderings alone: a strictly increasing
+    map of a variable leaves it unchanged bit for bit, and negating a
+    tie-free variable negates it exactly."""
+    rng = np.random.default_rng(67)
+    a = rng.standard_normal((40, 700)).astype(np.float32)
+    b = rng.standard_normal((30, 700)).astype(np.float32)
+    cube = a.astype(np.float64) ** 3
+    a3 = cube.astype(np.float32)
+    for r in range(a.shape[0]):     # the map keeps every sample distinct
+        assert np.unique(a3[r]).size == np.unique(a[r]).size == a.shape[1]
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    t0 = gt.kendall(da, db).cpu().numpy()
+    t1 = gt.kendall(torch.from_numpy(a3).cuda(), db).cpu().numpy()
+    t2 = gt.kendall(-da, db).cpu().numpy()
+    np.testing.assert_array_equal(t1, t0)
+    np.testing.assert_array_equal(t2, -t0)
