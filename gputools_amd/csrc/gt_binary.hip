@@ -249,15 +249,18 @@ constexpr int QSUPER = 4;                // super-tile side in tiles
 // 0.043 ms).
 constexpr int64_t QSMALL_ROUNDS = 4;
 
-// Tile geometry: QBM x QBM pairs on (QBM / 64)^2 waves of 64 x 64.
+// Tile geometry: QBM x QBM pairs on (QBM / 64)^2 waves of 64 x 64, plus
+// QX more B rows (i) so that each tile can finish the 128-byte line its
+// last row of every column ends in (the line-ownership rule below).
+constexpr int QX = 32;
 template <int QBM>
 struct QTile {
     static constexpr int WR = QBM / 64;            // waves per side
     static constexpr int NW = WR * WR;
     static constexpr int THREADS = NW * 64;
-    static constexpr int TERM = QBM * QBK;         // one side of one chunk
-    static constexpr int STAGE = 2 * TERM;
-    static constexpr int GPW = 2 * (QBM / 32) / NW;   // 32-row load groups per wave
+    static constexpr int TERM = QBM * QBK;         // the A side of one chunk
+    static constexpr int STAGE = (2 * QBM + QX) * QBK;
+    static constexpr int GPW = 2 * (QBM / 32) / NW;   // 32-row load groups per wave (wave 0: one more)
 };
 
 int64_t bytes_ld(int64_t k) { return (k + QBK - 1) / QBK * QBK; }
@@ -302,6 +305,23 @@ struct BinMfmaArgs {
     int64_t id_offset, ids_this;
 };
 
+// Line ownership.  Packed column j (the pairs (i, j), i > j) is one run of
+// the output.  A tile that wrote exactly its rows i0 .. i0 + QBM - 1 of it
+// would share the 128-byte lines at both ends with the tiles above and
+// below, which write them at other times (often through another XCD's L2),
+// so those lines would reach HBM as partial writes.  Here the 32-float line
+// holding row i of column j belongs to the tile whose rows hold
+// s(i) = max(j + 1, the first row of that line): tile I writes rows
+// [i0 + e_j, i0 + QBM + e_j) of column j, where e_j is the distance from row
+// i0 to the next line start (line_gap), except that the tile holding j + 1
+// starts at j + 1 and a tile below j + 1 writes nothing of the column.  The
+// QX extra B rows cover the overhang.  Only a line that holds the end of one
+// column and the start of the next is still written by two tiles.
+__device__ __forceinline__ unsigned line_gap(uint64_t float_index)
+{
+    return (unsigned)(0u - (unsigned)float_index) & 31u;
+}
+
 // Only called with at most SMALL_DIV_MAX features: the quotient is
 // small_quotient's.
 template <int QBM>
@@ -310,12 +330,18 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
 {
     typedef QTile<QBM> Q;
     constexpr int QTERM = Q::TERM, QSTAGE = Q::STAGE, NW = Q::NW, GPW = Q::GPW;
-    __shared__ __attribute__((aligned(1024))) unsigned char smem[QNSTAGE * QSTAGE + 2 * QBM * 4];
+    static_assert(GPW == 1 || GPW == 2, "one or two load groups per wave");
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[QNSTAGE * QSTAGE + (2 * QBM + QX) * 4];
     uint32_t *cA = reinterpret_cast<uint32_t *>(smem + QNSTAGE * QSTAGE);
     uint32_t *cB = cA + QBM;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wave / Q::WR, wc = wave % Q::WR;   // A rows (j) wr*64.., B rows (i) wc*64..
+    // the 2 QBM / 32 blocks of (QX extra B rows) x (32 A rows) go to waves
+    // two per SIMD: with 4 waves each takes its A block wc; with 16 the
+    // waves with wc / 2 == wr % 2 take A block wc % 2
+    const bool hasx = Q::WR == 2 || (wc >> 1) == (wr & 1);
+    const int xa = Q::WR == 2 ? wc : (wc & 1);
 
     const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
     const int64_t st = id / (QSUPER * QSUPER);
@@ -326,30 +352,31 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
     const int64_t J = (int64_t)SJ * QSUPER + w % QSUPER;
     if (I >= args.T || J > I) return;
     const int64_t i0 = I * QBM, j0 = J * QBM, n = args.n;
-    for (int t = tid; t < 2 * QBM; t += Q::THREADS) {
+    for (int t = tid; t < 2 * QBM + QX; t += Q::THREADS) {
         if (t < QBM)
             cA[t] = j0 + t < n ? args.cnt[j0 + t] : 0u;
         else
             cB[t - QBM] = i0 + t - QBM < n ? args.cnt[i0 + t - QBM] : 0u;
     }
 
-    // the 2 QBM / 32 groups of 32 rows go to the waves round robin: wave w
-    // fills groups w, w + NW, ... (group g: side g / (QBM / 32), rows
-    // 32 (g % (QBM / 32)) ..); rows past n are clamped to n - 1 (they only
-    // feed pairs never stored)
+    // 32-row load groups: g < QBM / 32 are A rows 32 g.., the rest B rows
+    // 32 (g - QBM / 32)..; group g lands at byte 1024 g of a stage.  Wave w
+    // fills groups w, w + NW, ..., and wave 0 also the last (the extra B
+    // rows).  Rows past n are clamped to n - 1 (they only feed pairs never
+    // stored).
     typedef __attribute__((address_space(3))) void lds_void_t;
     typedef const __attribute__((address_space(1))) void gbl_void_t;
     constexpr int GSIDE = QBM / 32;
-    const unsigned char *gsrc[GPW];
-    unsigned char *ldst[GPW];
+    const unsigned char *gsrc[GPW + 1];
+    unsigned char *ldst[GPW + 1];
 #pragma unroll
-    for (int q = 0; q < GPW; ++q) {
-        const int g = wave + q * NW;
+    for (int q = 0; q <= GPW; ++q) {
+        const int g = q < GPW ? wave + q * NW : 2 * GSIDE;
         const int rr = lane >> 1;
         const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
-        const int64_t r = ((g / GSIDE) ? i0 : j0) + (g % GSIDE) * 32 + rr;
+        const int64_t r = (g < GSIDE ? j0 + g * 32 : i0 + (g - GSIDE) * 32) + rr;
         gsrc[q] = args.ind + (r < n ? r : n - 1) * args.kb + half * 16;
-        ldst[q] = smem + (g / GSIDE) * QTERM + (g % GSIDE) * 1024;
+        ldst[q] = smem + g * 1024;
     }
     auto issue = [&](int chunk) {
 #pragma unroll
@@ -357,29 +384,43 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
             __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[q] + (int64_t)chunk * QBK),
                                              (lds_void_t *)(ldst[q] + (chunk % QNSTAGE) * QSTAGE), 16,
                                              0, 0);
+        if (wave == 0)
+            __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[GPW] + (int64_t)chunk * QBK),
+                                             (lds_void_t *)(ldst[GPW] + (chunk % QNSTAGE) * QSTAGE), 16,
+                                             0, 0);
     };
     const int lrow = lane & 31;
     const int roff = lrow * QBK + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
 
-    bin_i32x16_t acc[2][2];
+    bin_i32x16_t acc[2][2], accx;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int r = 0; r < 16; ++r) {
+        accx[r] = 0;
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0;
+            for (int b = 0; b < 2; ++b) acc[a][b][r] = 0;
+    }
 
     const int nchunks = (int)(args.kb / QBK);
     issue(0);
     if (nchunks > 1) issue(1);
     for (int c = 0; c < nchunks; ++c) {
         // chunk c landed when at most the next chunk's loads of this wave
-        // are outstanding (vector memory loads complete in order)
+        // are outstanding (vector memory loads complete in order); wave 0
+        // issues one load more per chunk
         if (c + 1 < nchunks) {
-            if (GPW == 1)
-                asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
-            else
-                asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+            if (wave == 0) {
+                if (GPW == 1)
+                    asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+                else
+                    asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+            } else {
+                if (GPW == 1)
+                    asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+                else
+                    asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+            }
         } else {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
@@ -399,23 +440,33 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
 #pragma unroll
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+        if (hasx) {
+            const bin_i32x4_t bx = *reinterpret_cast<const bin_i32x4_t *>(sg + QTERM + QBM * QBK);
+            accx = __builtin_amdgcn_mfma_i32_32x32x32_i8(xa ? av[1] : av[0], bx, accx, 0, 0, 0);
+        }
     }
     __syncthreads();   // cA / cB visible
 
     // 32x32 accumulator layout: element r of lane l is A row 8(r/4) +
     // 4(l/32) + r%4 (j) and B row l%32 (i): lanes 0..31 store 32 consecutive
-    // i of one j.
+    // i of one j.  accx holds B rows QBM + l%32 of the wave's A block xa.
     const int li0 = wc * 64 + (lane & 31);
+    const int lx = lane & 31;
     uint32_t pb[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) pb[b] = cB[li0 + b * 32];
-    if (I > J && i0 + QBM <= n) {
-        // every pair valid, i > j: row offsets from the wave's first row jw
-        // fit in 32 bits (n < 2^24)
+    const uint32_t pbx = cB[QBM + lx];
+    const uint64_t fout = reinterpret_cast<uintptr_t>(args.out) >> 2;
+    if (I > J && i0 + QBM + QX <= n) {
+        // every computed pair valid, i > j: row offsets from the wave's
+        // first row jw fit in 32 bits (n < 2^24)
         const int64_t jw = j0 + wr * 64;
         float *const outw = args.out + packed_base(n, jw) + i0;
+        const unsigned fw = (unsigned)fout + (unsigned)(packed_base(n, jw) + i0);
         const unsigned cw = (unsigned)(n - 1 - jw);
         const bool hi = (lane >> 5) != 0;
+        // column j = i0 - 1 (tile I = J + 1, last column of J) starts at i0
+        const int lastc = (i0 - j0 == QBM) ? QBM - 1 - wr * 64 - (hi ? 4 : 0) : -1;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -423,11 +474,17 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
                 const unsigned dc = a * 32 + 8 * (r >> 2) + (r & 3);
                 const unsigned base_u = dc * cw - dc * (dc + 1) / 2;
                 const unsigned step_u = 4 * cw - 4 * dc - 10;
-                const unsigned ro = base_u + (hi ? step_u : 0u) + (unsigned)li0;
+                const unsigned rc = base_u + (hi ? step_u : 0u);   // column j at row i0
+                const unsigned e = (0u - (fw + rc)) & 31u;
+                const unsigned lo = (int)dc == lastc ? 0u : e;
                 const uint32_t pa = cA[wr * 64 + (int)dc + (hi ? 4 : 0)];
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                    outw[ro + (unsigned)(b * 32)] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
+                    if ((unsigned)(li0 + b * 32) >= lo)
+                        outw[rc + (unsigned)(li0 + b * 32)] =
+                            binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
+                if (hasx && a == xa && (unsigned)lx < e)
+                    outw[rc + (unsigned)(QBM + lx)] = binary_value<true>(pa + pbx, (uint32_t)accx[r]);
             }
         return;
     }
@@ -437,13 +494,22 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
         for (int r = 0; r < 16; ++r) {
             const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const int64_t j = j0 + lj;
-            if (j >= n) continue;
+            if (j >= n || j + 1 >= i0 + QBM) continue;
+            const int64_t pbj = packed_base(n, j);
+            const int64_t e = line_gap(fout + (uint64_t)(pbj + i0));
+            const int64_t lo = j + 1 >= i0 ? j + 1 : i0 + e;
+            const int64_t hi = i0 + QBM + e < n ? i0 + QBM + e : n;
             const uint32_t pa = cA[lj];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 const int64_t i = i0 + li0 + b * 32;
-                if (i < n && i > j)
-                    args.out[packed_base(n, j) + i] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
+                if (i >= lo && i < hi)
+                    args.out[pbj + i] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
+            }
+            if (hasx && a == xa) {
+                const int64_t i = i0 + QBM + lx;
+                if (i >= lo && i < hi)
+                    args.out[pbj + i] = binary_value<true>(pa + pbx, (uint32_t)accx[r]);
             }
         }
 }
