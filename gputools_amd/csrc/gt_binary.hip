@@ -54,6 +54,19 @@ __device__ __forceinline__ float binary_value(uint32_t pa_plus_pb, uint32_t both
     return no == 0 ? 0.0f : q;
 }
 
+// The same value with y = RN(1/no) read from a table (lut[0] = 0: no = 0
+// implies nx = 0, and the sequence then gives +0).  RN(1/no) is one of the
+// reciprocals tests/exhaustive/small_div.c checks.
+__device__ __forceinline__ float binary_value_lut(uint32_t pa_plus_pb, uint32_t both, const float *lut)
+{
+    const uint32_t no = pa_plus_pb - both, nx = no - both;
+    const float y = lut[no];
+    const float a = (float)nx, b = (float)no;
+    const float q0 = __fmul_rn(a, y);
+    const float r = fmaf(-q0, b, a);
+    return fmaf(r, y, q0);
+}
+
 __global__ __launch_bounds__(256) void binary_pack_kernel(const float *__restrict__ x, int64_t n,
                                                           int64_t k, int64_t ldx,
                                                           uint32_t *__restrict__ bits, int64_t ldw,
@@ -231,37 +244,71 @@ __global__ __launch_bounds__(BTHREADS) void binary_tile_kernel(BinaryArgs args)
 // Packed whole-matrix requests: the counts on the matrix cores.  Each row is
 // one 0/1 byte per feature (zero-padded to a multiple of 32) and its count of
 // ones; v_mfma_i32_32x32x32_i8 sums the products of two rows' bytes exactly,
-// which is the number of features on in both.  The tiles are 256 x 256 on 16
-// waves (64 x 64 pairs each); chunks of 32 features (32 B per row) reach LDS
-// through global_load_lds into a three-stage ring, 16-byte halves swapped by
-// ((R >> 2) ^ (R >> 3)) & 1 so that the operand reads of consecutive rows
-// hit distinct banks.
+// which is the number of features on in both.  The tiles are 256 x 256 (two
+// workgroups of 8 waves) or 128 x 128 (4 waves); chunks of 32 features (32 B
+// per row) reach LDS through global_load_lds into a ring of NSTAGE stages,
+// 16-byte halves swapped by ((R >> 2) ^ (R >> 3)) & 1 so that the operand
+// reads of consecutive rows hit distinct banks.
 // ---------------------------------------------------------------------------
 typedef int bin_i32x4_t __attribute__((ext_vector_type(4)));
 typedef int bin_i32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int QBK = 32;                  // features per chunk (bytes per row)
-constexpr int QNSTAGE = 3;
 constexpr int QSUPER = 4;                // super-tile side in tiles
-// Tile side: 256 x 256 pairs (16 waves) for large n; 128 x 128 (4 waves,
-// four workgroups per CU) while the 256 schedule has fewer than QSMALL_ROUNDS
-// rounds of 256 CUs (65536 x 256: 2.73 vs 3.07 ms; 8192 x 256: 0.049 vs
-// 0.043 ms).
+// Tile side: 256 x 256 pairs for large n; 128 x 128 (four workgroups per
+// CU) while the 256 schedule has fewer than QSMALL_ROUNDS rounds of 256 CUs
+// (measured with the first matrix-core form, 64 x 64 pairs per wave:
+// 65536 x 256 2.73 vs 3.07 ms, 8192 x 256 0.049 vs 0.043 ms).
 constexpr int64_t QSMALL_ROUNDS = 4;
 
-// Tile geometry: QBM x QBM pairs on (QBM / 64)^2 waves of 64 x 64, plus
-// QX more B rows (i) so that each tile can finish the 128-byte line its
-// last row of every column ends in (the line-ownership rule below).
+// Tile geometry.  Square tiles of QB x QB pairs (B rows i, A rows j) are
+// split along j into QB / QA workgroups of QA columns each.  A workgroup
+// computes its QA columns against QB + QX rows: the QX extra rows let it
+// finish the 128-byte line that its last row of every column ends in (the
+// line-ownership rule below).  Wave (wj, wi) holds A rows 32 wj .. 32 wj + 31
+// against B rows 128 wi .. 128 wi + 159: five 32 x 32 blocks, the fifth
+// shared with the next wave (or the extra rows).  Two 8-wave workgroups per
+// 256 tile ran as fast as one of 16 waves (2.52 vs 2.51 ms at 65536 x 256)
+// with less LDS per workgroup.
 constexpr int QX = 32;
-template <int QBM>
+constexpr int QXB = 5;                          // B blocks per wave
+// Reciprocal table entries (no = 0 .. 2 kb) when 2 kb < QLUT: an LDS read
+// instead of v_rcp_f32 (a quarter-rate instruction) and the no = 0 select.
+constexpr int QLUT = 2048;
+template <int QA, int QB>
 struct QTile {
-    static constexpr int WR = QBM / 64;            // waves per side
-    static constexpr int NW = WR * WR;
+    static constexpr int WJ = QA / 32;             // waves along j
+    static constexpr int WI = QB / 128;            // waves along i
+    static constexpr int NW = WJ * WI;
     static constexpr int THREADS = NW * 64;
-    static constexpr int TERM = QBM * QBK;         // the A side of one chunk
-    static constexpr int STAGE = (2 * QBM + QX) * QBK;
-    static constexpr int GPW = 2 * (QBM / 32) / NW;   // 32-row load groups per wave (wave 0: one more)
+    static constexpr int PARTS = QB / QA;          // workgroups per square tile
+    static constexpr int GA = QA / 32;             // 32-row load groups: A side,
+    static constexpr int G = GA + (QB + QX) / 32;  // then B side
+    static constexpr int GMAX = (G + NW - 1) / NW; // groups of wave 0
+    static constexpr int TERM = QA * QBK;          // the A side of one chunk
+    static constexpr int STAGE = G * 32 * QBK;
+    static constexpr int NSTAGE = 3;               // ring stages
 };
+
+// s_waitcnt vmcnt(v) for a wave-uniform v (larger v: waits for everything)
+__device__ __forceinline__ void wait_vmcnt(int v)
+{
+    switch (v) {
+    case 1: asm volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+    case 2: asm volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+    case 3: asm volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+    case 4: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+    case 5: asm volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    case 7: asm volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    case 9: asm volatile("s_waitcnt vmcnt(9)" ::: "memory"); break;
+    case 10: asm volatile("s_waitcnt vmcnt(10)" ::: "memory"); break;
+    case 11: asm volatile("s_waitcnt vmcnt(11)" ::: "memory"); break;
+    case 12: asm volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
 
 int64_t bytes_ld(int64_t k) { return (k + QBK - 1) / QBK * QBK; }
 
@@ -306,44 +353,52 @@ struct BinMfmaArgs {
 };
 
 // Line ownership.  Packed column j (the pairs (i, j), i > j) is one run of
-// the output.  A tile that wrote exactly its rows i0 .. i0 + QBM - 1 of it
+// the output.  A tile that wrote exactly its rows i0 .. i0 + QB - 1 of it
 // would share the 128-byte lines at both ends with the tiles above and
 // below, which write them at other times (often through another XCD's L2),
 // so those lines would reach HBM as partial writes.  Here the 32-float line
 // holding row i of column j belongs to the tile whose rows hold
 // s(i) = max(j + 1, the first row of that line): tile I writes rows
-// [i0 + e_j, i0 + QBM + e_j) of column j, where e_j is the distance from row
+// [i0 + e_j, i0 + QB + e_j) of column j, where e_j is the distance from row
 // i0 to the next line start (line_gap), except that the tile holding j + 1
 // starts at j + 1 and a tile below j + 1 writes nothing of the column.  The
 // QX extra B rows cover the overhang.  Only a line that holds the end of one
 // column and the start of the next is still written by two tiles.
+//
+// Inside a tile, wave wi writes rows [128 wi + e_j, 128 wi + 128 + e_j) of
+// its columns as four whole lines: each half-wave rotates its five blocks'
+// values down by e_j lanes (ds_bpermute), so one store instruction writes
+// one aligned line per half-wave.
 __device__ __forceinline__ unsigned line_gap(uint64_t float_index)
 {
     return (unsigned)(0u - (unsigned)float_index) & 31u;
 }
 
+__device__ __forceinline__ float lane_fetch(float v, int src_byte)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
+}
+
 // Only called with at most SMALL_DIV_MAX features: the quotient is
 // small_quotient's.
-template <int QBM>
-__global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) void binary_mfma_kernel(
+template <int QA, int QB, bool LUT>
+__global__ __launch_bounds__((QTile<QA, QB>::THREADS), (1024 / QTile<QA, QB>::THREADS)) void binary_mfma_kernel(
     BinMfmaArgs args)
 {
-    typedef QTile<QBM> Q;
-    constexpr int QTERM = Q::TERM, QSTAGE = Q::STAGE, NW = Q::NW, GPW = Q::GPW;
-    static_assert(GPW == 1 || GPW == 2, "one or two load groups per wave");
-    __shared__ __attribute__((aligned(1024))) unsigned char smem[QNSTAGE * QSTAGE + (2 * QBM + QX) * 4];
-    uint32_t *cA = reinterpret_cast<uint32_t *>(smem + QNSTAGE * QSTAGE);
-    uint32_t *cB = cA + QBM;
+    typedef QTile<QA, QB> Q;
+    constexpr int QTERM = Q::TERM, QSTAGE = Q::STAGE, NW = Q::NW, NS = Q::NSTAGE, G = Q::G, GA = Q::GA;
+    static_assert((NS - 2) * Q::GMAX <= 12, "wait_vmcnt covers the loads in flight");
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[NS * QSTAGE + (QA + QB + QX) * 4];
+    uint32_t *cA = reinterpret_cast<uint32_t *>(smem + NS * QSTAGE);
+    uint32_t *cB = cA + QA;
+    __shared__ float lut[LUT ? QLUT : 1];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wr = wave / Q::WR, wc = wave % Q::WR;   // A rows (j) wr*64.., B rows (i) wc*64..
-    // the 2 QBM / 32 blocks of (QX extra B rows) x (32 A rows) go to waves
-    // two per SIMD: with 4 waves each takes its A block wc; with 16 the
-    // waves with wc / 2 == wr % 2 take A block wc % 2
-    const bool hasx = Q::WR == 2 || (wc >> 1) == (wr & 1);
-    const int xa = Q::WR == 2 ? wc : (wc & 1);
+    const int wj = wave % Q::WJ, wi = wave / Q::WJ;   // A rows (j) 32 wj.., B rows (i) 128 wi..
 
-    const int64_t id = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int64_t gid = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    const int part = (int)(gid % Q::PARTS);
+    const int64_t id = gid / Q::PARTS;
     const int64_t st = id / (QSUPER * QSUPER);
     const int w = (int)(id % (QSUPER * QSUPER));
     int SI, SJ;
@@ -351,167 +406,145 @@ __global__ __launch_bounds__(QTile<QBM>::THREADS, 1024 / QTile<QBM>::THREADS) vo
     const int64_t I = (int64_t)SI * QSUPER + w / QSUPER;
     const int64_t J = (int64_t)SJ * QSUPER + w % QSUPER;
     if (I >= args.T || J > I) return;
-    const int64_t i0 = I * QBM, j0 = J * QBM, n = args.n;
-    for (int t = tid; t < 2 * QBM + QX; t += Q::THREADS) {
-        if (t < QBM)
+    const int64_t i0 = I * QB, j0 = J * QB + part * QA, n = args.n;
+    if (j0 >= n) return;
+    for (int t = tid; t < QA + QB + QX; t += Q::THREADS) {
+        if (t < QA)
             cA[t] = j0 + t < n ? args.cnt[j0 + t] : 0u;
         else
-            cB[t - QBM] = i0 + t - QBM < n ? args.cnt[i0 + t - QBM] : 0u;
+            cB[t - QA] = i0 + t - QA < n ? args.cnt[i0 + t - QA] : 0u;
     }
+    if (LUT)
+        for (int t = tid; t <= 2 * args.kb; t += Q::THREADS) lut[t] = t ? 1.0f / (float)t : 0.0f;
+    auto value = [&](uint32_t pa_plus_pb, uint32_t both) {
+        return LUT ? binary_value_lut(pa_plus_pb, both, lut) : binary_value<true>(pa_plus_pb, both);
+    };
 
-    // 32-row load groups: g < QBM / 32 are A rows 32 g.., the rest B rows
-    // 32 (g - QBM / 32)..; group g lands at byte 1024 g of a stage.  Wave w
-    // fills groups w, w + NW, ..., and wave 0 also the last (the extra B
-    // rows).  Rows past n are clamped to n - 1 (they only feed pairs never
-    // stored).
+    // 32-row load groups: g < GA are A rows 32 g.., the rest B rows
+    // 32 (g - GA)..; group g lands at byte 1024 g of a stage.  Wave w fills
+    // groups w, w + NW, ... below G.  Rows past n are clamped to n - 1 (they
+    // only feed pairs never stored).
     typedef __attribute__((address_space(3))) void lds_void_t;
     typedef const __attribute__((address_space(1))) void gbl_void_t;
-    constexpr int GSIDE = QBM / 32;
-    const unsigned char *gsrc[GPW + 1];
-    unsigned char *ldst[GPW + 1];
+    const unsigned char *gsrc[Q::GMAX];
+    unsigned char *ldst[Q::GMAX];
 #pragma unroll
-    for (int q = 0; q <= GPW; ++q) {
-        const int g = q < GPW ? wave + q * NW : 2 * GSIDE;
+    for (int q = 0; q < Q::GMAX; ++q) {
+        const int g = wave + q * NW < G ? wave + q * NW : G - 1;
         const int rr = lane >> 1;
         const int half = (lane & 1) ^ (((rr >> 2) ^ (rr >> 3)) & 1);
-        const int64_t r = (g < GSIDE ? j0 + g * 32 : i0 + (g - GSIDE) * 32) + rr;
+        const int64_t r = (g < GA ? j0 + g * 32 : i0 + (g - GA) * 32) + rr;
         gsrc[q] = args.ind + (r < n ? r : n - 1) * args.kb + half * 16;
         ldst[q] = smem + g * 1024;
     }
     auto issue = [&](int chunk) {
 #pragma unroll
-        for (int q = 0; q < GPW; ++q)
-            __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[q] + (int64_t)chunk * QBK),
-                                             (lds_void_t *)(ldst[q] + (chunk % QNSTAGE) * QSTAGE), 16,
-                                             0, 0);
-        if (wave == 0)
-            __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[GPW] + (int64_t)chunk * QBK),
-                                             (lds_void_t *)(ldst[GPW] + (chunk % QNSTAGE) * QSTAGE), 16,
-                                             0, 0);
+        for (int q = 0; q < Q::GMAX; ++q)
+            if (wave + q * NW < G)
+                __builtin_amdgcn_global_load_lds((gbl_void_t *)(gsrc[q] + (int64_t)chunk * QBK),
+                                                 (lds_void_t *)(ldst[q] + (chunk % NS) * QSTAGE), 16, 0,
+                                                 0);
     };
     const int lrow = lane & 31;
     const int roff = lrow * QBK + (((lane >> 5) ^ (((lrow >> 2) ^ (lrow >> 3)) & 1)) * 16);
 
-    bin_i32x16_t acc[2][2], accx;
+    bin_i32x16_t acc[QXB];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        accx[r] = 0;
+    for (int b = 0; b < QXB; ++b)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b][r] = 0;
-    }
+        for (int r = 0; r < 16; ++r) acc[b][r] = 0;
 
     const int nchunks = (int)(args.kb / QBK);
-    issue(0);
-    if (nchunks > 1) issue(1);
+    const int lpc = (G - wave + NW - 1) / NW;   // this wave's loads per chunk
+    for (int c = 0; c < NS - 1 && c < nchunks; ++c) issue(c);
     for (int c = 0; c < nchunks; ++c) {
-        // chunk c landed when at most the next chunk's loads of this wave
-        // are outstanding (vector memory loads complete in order); wave 0
-        // issues one load more per chunk
-        if (c + 1 < nchunks) {
-            if (wave == 0) {
-                if (GPW == 1)
-                    asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
-                else
-                    asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
-            } else {
-                if (GPW == 1)
-                    asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
-                else
-                    asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
-            }
-        } else {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        }
+        // chunks c .. c + ahead are in flight; chunk c landed when at most
+        // the later ones' loads of this wave are outstanding (vector memory
+        // loads complete in order)
+        const int ahead = (nchunks - 1 - c) < NS - 2 ? (nchunks - 1 - c) : NS - 2;
+        wait_vmcnt(ahead * lpc);
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        if (c + 2 < nchunks) issue(c + 2);
-        const unsigned char *sg = smem + (c % QNSTAGE) * QSTAGE + roff;
-        bin_i32x4_t av[2], bv[2];
+        // the stage read in the previous iteration, which every wave finished
+        if (c + NS - 1 < nchunks) issue(c + NS - 1);
+        const unsigned char *sg = smem + (c % NS) * QSTAGE + roff;
+        const bin_i32x4_t av = *reinterpret_cast<const bin_i32x4_t *>(sg + wj * 32 * QBK);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-            av[a] = *reinterpret_cast<const bin_i32x4_t *>(sg + (wr * 64 + a * 32) * QBK);
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-            bv[b] = *reinterpret_cast<const bin_i32x4_t *>(sg + QTERM + (wc * 64 + b * 32) * QBK);
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
-        if (hasx) {
-            const bin_i32x4_t bx = *reinterpret_cast<const bin_i32x4_t *>(sg + QTERM + QBM * QBK);
-            accx = __builtin_amdgcn_mfma_i32_32x32x32_i8(xa ? av[1] : av[0], bx, accx, 0, 0, 0);
+        for (int b = 0; b < QXB; ++b) {
+            const bin_i32x4_t bv =
+                *reinterpret_cast<const bin_i32x4_t *>(sg + QTERM + (wi * 128 + b * 32) * QBK);
+            acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[b], 0, 0, 0);
         }
     }
-    __syncthreads();   // cA / cB visible
+    __syncthreads();   // cA, cB and lut visible
 
     // 32x32 accumulator layout: element r of lane l is A row 8(r/4) +
-    // 4(l/32) + r%4 (j) and B row l%32 (i): lanes 0..31 store 32 consecutive
-    // i of one j.  accx holds B rows QBM + l%32 of the wave's A block xa.
-    const int li0 = wc * 64 + (lane & 31);
+    // 4(l/32) + r%4 (j) and B row l%32 (i): lanes 0..31 hold 32 consecutive
+    // i of one j, block b rows 128 wi + 32 b.
     const int lx = lane & 31;
-    uint32_t pb[2];
+    const bool hi = (lane >> 5) != 0;
+    const int lib = wi * 128 + lx;
+    uint32_t pb[QXB];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) pb[b] = cB[li0 + b * 32];
-    const uint32_t pbx = cB[QBM + lx];
+    for (int b = 0; b < QXB; ++b) pb[b] = cB[lib + b * 32];
     const uint64_t fout = reinterpret_cast<uintptr_t>(args.out) >> 2;
-    if (I > J && i0 + QBM + QX <= n) {
+    if (I > J && i0 + QB + QX <= n) {
         // every computed pair valid, i > j: row offsets from the wave's
         // first row jw fit in 32 bits (n < 2^24)
-        const int64_t jw = j0 + wr * 64;
+        const int64_t jw = j0 + wj * 32;
         float *const outw = args.out + packed_base(n, jw) + i0;
         const unsigned fw = (unsigned)fout + (unsigned)(packed_base(n, jw) + i0);
         const unsigned cw = (unsigned)(n - 1 - jw);
-        const bool hi = (lane >> 5) != 0;
-        // column j = i0 - 1 (tile I = J + 1, last column of J) starts at i0
-        const int lastc = (i0 - j0 == QBM) ? QBM - 1 - wr * 64 - (hi ? 4 : 0) : -1;
+        // column j = i0 - 1 (the last column of the tile row above) starts
+        // at i0: wave wi = 0 also writes its rows before the first line start
+        const int lastc = (j0 + QA == i0 && wi == 0) ? QA - 1 - wj * 32 - (hi ? 4 : 0) : -1;
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int r = 0; r < 16; ++r) {
+            const unsigned dc = 8 * (r >> 2) + (r & 3);
+            const unsigned base_u = dc * cw - dc * (dc + 1) / 2;
+            const unsigned step_u = 4 * cw - 4 * dc - 10;
+            const unsigned rc = base_u + (hi ? step_u : 0u);   // column j at row i0
+            const unsigned e = (0u - (fw + rc)) & 31u;
+            const uint32_t pa = cA[wj * 32 + (int)dc + (hi ? 4 : 0)];
+            float v[QXB];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const unsigned dc = a * 32 + 8 * (r >> 2) + (r & 3);
-                const unsigned base_u = dc * cw - dc * (dc + 1) / 2;
-                const unsigned step_u = 4 * cw - 4 * dc - 10;
-                const unsigned rc = base_u + (hi ? step_u : 0u);   // column j at row i0
-                const unsigned e = (0u - (fw + rc)) & 31u;
-                const unsigned lo = (int)dc == lastc ? 0u : e;
-                const uint32_t pa = cA[wr * 64 + (int)dc + (hi ? 4 : 0)];
+            for (int b = 0; b < QXB; ++b) v[b] = value(pa + pb[b], (uint32_t)acc[b][r]);
+            // byte offsets from the wave-uniform outw stay in 32 bits
+            char *const ob = reinterpret_cast<char *>(outw);
+            if ((int)dc == lastc && (unsigned)lx < e)
+                *reinterpret_cast<float *>(ob + 4u * (rc + (unsigned)lx)) = v[0];
+            // line m, lane lx: row 128 wi + e + 32 m + lx, held by lane
+            // (lx + e) % 32 in block m, or in block m + 1 when lx + e >= 32;
+            // so source lane l sends block m + 1 if l < e, else block m
+            const int src = ((lane & 32) | (int)(((unsigned)lx + e) & 31u)) << 2;
+            const bool sends_next = (unsigned)lx < e;
+            const unsigned lb = rc + (unsigned)(wi * 128) + e + (unsigned)lx;
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    if ((unsigned)(li0 + b * 32) >= lo)
-                        outw[rc + (unsigned)(li0 + b * 32)] =
-                            binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
-                if (hasx && a == xa && (unsigned)lx < e)
-                    outw[rc + (unsigned)(QBM + lx)] = binary_value<true>(pa + pbx, (uint32_t)accx[r]);
-            }
+            for (int m = 0; m < 4; ++m)
+                *reinterpret_cast<float *>(ob + 4u * (lb + 32u * m)) =
+                    lane_fetch(sends_next ? v[m + 1] : v[m], src);
+        }
         return;
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int r = 0; r < 16; ++r) {
+        const int lj = wj * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t j = j0 + lj;
+        if (j >= n || j + 1 >= i0 + QB) continue;
+        const int64_t pbj = packed_base(n, j);
+        const int64_t e = line_gap(fout + (uint64_t)(pbj + i0));
+        const int64_t lo = j + 1 >= i0 ? j + 1 : i0 + e;
+        const int64_t hi_ = i0 + QB + e < n ? i0 + QB + e : n;
+        const uint32_t pa = cA[lj];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int lj = wr * 64 + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int64_t j = j0 + lj;
-            if (j >= n || j + 1 >= i0 + QBM) continue;
-            const int64_t pbj = packed_base(n, j);
-            const int64_t e = line_gap(fout + (uint64_t)(pbj + i0));
-            const int64_t lo = j + 1 >= i0 ? j + 1 : i0 + e;
-            const int64_t hi = i0 + QBM + e < n ? i0 + QBM + e : n;
-            const uint32_t pa = cA[lj];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int64_t i = i0 + li0 + b * 32;
-                if (i >= lo && i < hi)
-                    args.out[pbj + i] = binary_value<true>(pa + pb[b], (uint32_t)acc[a][b][r]);
-            }
-            if (hasx && a == xa) {
-                const int64_t i = i0 + QBM + lx;
-                if (i >= lo && i < hi)
-                    args.out[pbj + i] = binary_value<true>(pa + pbx, (uint32_t)accx[r]);
-            }
+        for (int b = 0; b < QXB; ++b) {
+            // the last block is the next wave's first, except for the last wave
+            if (b == QXB - 1 && wi != Q::WI - 1) continue;
+            const int64_t i = i0 + lib + b * 32;
+            if (i >= lo && i < hi_)
+                args.out[pbj + i] = value(pa + pb[b], (uint32_t)acc[b][r]);
         }
+    }
 }
 
 template <bool FASTDIV>
@@ -525,18 +558,19 @@ void launch_layout(int layout, dim3 grid, dim3 block, hipStream_t stream, const 
         hipLaunchKernelGGL((binary_tile_kernel<OUT_CROSS, FASTDIV>), grid, block, 0, stream, args);
 }
 
-template <int QBM>
+template <int QA, int QB, bool LUT>
 hipError_t launch_mfma_tiles(BinMfmaArgs &args, hipStream_t stream)
 {
-    args.T = (args.n + QBM - 1) / QBM;
+    typedef QTile<QA, QB> Q;
+    args.T = (args.n + QB - 1) / QB;
     const int64_t S = (args.T + QSUPER - 1) / QSUPER;
-    const int64_t ids = S * (S + 1) / 2 * QSUPER * QSUPER;
-    const int64_t max_ids = int64_t(1) << 23;
+    const int64_t ids = S * (S + 1) / 2 * QSUPER * QSUPER * Q::PARTS;
+    const int64_t max_ids = int64_t(1) << 23;   // a multiple of PARTS
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
         args.ids_this = (ids - off) < max_ids ? (ids - off) : max_ids;
-        hipLaunchKernelGGL((binary_mfma_kernel<QBM>), dim3((unsigned)args.ids_this),
-                           dim3(QTile<QBM>::THREADS), 0, stream, args);
+        hipLaunchKernelGGL((binary_mfma_kernel<QA, QB, LUT>), dim3((unsigned)args.ids_this), dim3(Q::THREADS),
+                           0, stream, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -589,10 +623,12 @@ hipError_t launch_binary_self(int layout, const float *x, int64_t n, int64_t k, 
     args.n = n;
     args.kb = kb;
     args.out = out;
+    const bool lut = 2 * kb < QLUT;
     const int64_t t256 = (n + 255) / 256;
     if (t256 * (t256 + 1) / 2 < QSMALL_ROUNDS * 256)
-        return launch_mfma_tiles<128>(args, stream);
-    return launch_mfma_tiles<256>(args, stream);
+        return lut ? launch_mfma_tiles<128, 128, true>(args, stream)
+                   : launch_mfma_tiles<128, 128, false>(args, stream);
+    return lut ? launch_mfma_tiles<128, 256, true>(args, stream) : launch_mfma_tiles<128, 256, false>(args, stream);
 }
 
 int64_t binary_ldw(int64_t k) { return binary_words(k) + 1; }

@@ -110,16 +110,22 @@ def test_many_features(n, k, method):
     assert_parity(run(x, method, p), orc.dist(x, method, p), method)
 
 
-def test_binary_matrix_core_large_tiles():
+@pytest.mark.parametrize("k", [100, 600])
+def test_binary_matrix_core_large_tiles(k):
     """12000 points: the packed binary path counts on the matrix cores in
-    256 x 256 tiles (fewer points use 128 x 128 tiles).  Whole rows across
-    the tile edges and random pairs, bit-exact."""
-    n, k = 12000, 100
+    256 x 256 tiles, two workgroups of 128 columns each (fewer points use
+    128 x 128 tiles); k = 100 takes the reciprocal table, k = 600 v_rcp_f32.
+    Every element written (the output starts as NaN: the tiles' line
+    ownership covers each value once); whole rows across the tile edges and
+    random pairs bit-exact."""
+    n = 12000
     x = wl.bernoulli(n, k, seed=25)
     x[300] = 0.0
-    print("case binary matrix-core 12000x100", flush=True)
-    got = gt.dist(dev(x), "binary")
+    print("case binary matrix-core 12000 x", k, flush=True)
+    got = torch.full((n * (n - 1) // 2,), float("nan"), device="cuda")
+    gt.dist(dev(x), "binary", out=got)
     torch.cuda.synchronize()
+    assert not torch.isnan(got).any()
     rng = np.random.default_rng(3)
     jj = np.concatenate([np.repeat([0, 255, 256, 300, 11775, 11998], n),
                          rng.integers(0, n, 200000)])
