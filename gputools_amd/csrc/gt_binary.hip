@@ -350,6 +350,7 @@ struct BinMfmaArgs {
     float *out;                 // packed
     int64_t T;                  // tiles per side
     int64_t id_offset, ids_this;
+    int64_t stagger_below;      // odd workgroups below this id start late (first round)
 };
 
 // Line ownership.  Packed column j (the pairs (i, j), i > j) is one run of
@@ -397,6 +398,15 @@ __global__ __launch_bounds__((QTile<QA, QB>::THREADS), (1024 / QTile<QA, QB>::TH
     const int wj = wave % Q::WJ, wi = wave / Q::WJ;   // A rows (j) 32 wj.., B rows (i) 128 wi..
 
     const int64_t gid = args.id_offset + xcd_remap(blockIdx.x, args.ids_this);
+    // Two 8-wave workgroups share a CU and have the same work, so if they
+    // start together they stay in step, and both store at once and then both
+    // run the matrix cores.  The odd workgroups of the first round start
+    // about 0.4 of a tile late, which keeps the pairs offset afterwards
+    // (65536 x 256, A/B on one box: 2.12 vs 2.30 ms).
+    if ((blockIdx.x & 1) && (int64_t)blockIdx.x < args.stagger_below) {
+        __builtin_amdgcn_s_sleep(127);
+        __builtin_amdgcn_s_sleep(127);
+    }
     const int part = (int)(gid % Q::PARTS);
     const int64_t id = gid / Q::PARTS;
     const int64_t st = id / (QSUPER * QSUPER);
@@ -565,6 +575,14 @@ hipError_t launch_mfma_tiles(BinMfmaArgs &args, hipStream_t stream)
     args.T = (args.n + QB - 1) / QB;
     const int64_t S = (args.T + QSUPER - 1) / QSUPER;
     const int64_t ids = S * (S + 1) / 2 * QSUPER * QSUPER * Q::PARTS;
+    args.stagger_below = 0;
+    if (Q::PARTS == 2) {   // two workgroups per CU: stagger the first round
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        args.stagger_below = 2 * (int64_t)cus;
+    }
     const int64_t max_ids = int64_t(1) << 23;   // a multiple of PARTS
     for (int64_t off = 0; off < ids; off += max_ids) {
         args.id_offset = off;
