@@ -138,6 +138,37 @@ def test_binary_matrix_core_large_tiles(k):
     np.testing.assert_array_equal(vals, orc.dist_pairs(x, hi, lo, "binary").astype(np.float32))
 
 
+@pytest.mark.parametrize("n", [3000, 11700])
+@pytest.mark.parametrize("k", [480, 481])
+def test_binary_quotient_forms_at_their_boundary(n, k):
+    """k = 480 is the widest row the matrix-core tiles serve with the LDS
+    reciprocal table (2 x 480 padded features < 1024); k = 481 pads to 512
+    and takes v_rcp_f32.  n = 3000 runs the 128 x 128 tiles (every pair
+    compared), n = 11700 the 256 x 256 tiles with their staggered start and a
+    ragged last tile row (the output starts as NaN: every element written;
+    tile-edge columns and random pairs compared).  Bit-exact."""
+    x = wl.bernoulli(n, k, seed=n + k)
+    x[7] = 0.0
+    print("case binary quotient boundary", n, k, flush=True)
+    got = torch.full((n * (n - 1) // 2,), float("nan"), device="cuda")
+    gt.dist(dev(x), "binary", out=got)
+    torch.cuda.synchronize()
+    assert not torch.isnan(got).any()
+    if n <= 3000:
+        np.testing.assert_array_equal(got.cpu().numpy(), orc.dist(x, "binary").astype(np.float32))
+        return
+    rng = np.random.default_rng(n + k)
+    edge = [0, 7, 127, 128, 255, 256, 11263, 11264, 11519, 11520, n - 2]
+    jj = np.concatenate([np.repeat(edge, n), rng.integers(0, n, 100000)])
+    ii = np.concatenate([np.tile(np.arange(n), len(edge)), rng.integers(0, n, 100000)])
+    keep = ii != jj
+    ii, jj = ii[keep], jj[keep]
+    hi, lo = np.maximum(ii, jj), np.minimum(ii, jj)
+    idx = lo * n - lo * (lo + 1) // 2 + (hi - lo - 1)
+    vals = got[torch.from_numpy(idx).cuda()].cpu().numpy()
+    np.testing.assert_array_equal(vals, orc.dist_pairs(x, hi, lo, "binary").astype(np.float32))
+
+
 @pytest.mark.parametrize("layout", ["packed", "full"])
 def test_binary_beyond_the_fast_quotient(layout):
     """Past 131072 features the binary ratio takes the IEEE divide instead of
